@@ -1,0 +1,353 @@
+"""Python host layer over librbf_amd.so (the C ABI of include/rbf_assembly.h).
+
+Mirrors the reference's host interface (ivan-pi/gpu-assembly) so code and
+tests read the same:
+
+    Permutation            rbf_reorder.h:69-157   (gather form, map/inv/then)
+    morton_keys, hilbert_keys, morton_order, hilbert_order
+                           rbf_reorder.h:193-229
+    renumber_stencils      rbf_reorder.h:278-288
+    make_row_ptr           rbf_reorder.h:316-322
+    NodeSet                rbf_nodeset.h:46-235   (renumber chain, file_order,
+                                                   stencils(k))
+    Plan / assemble        interp_config + assemble_interp_weights /
+                           assemble_laplace_weights (rbf_operators.h:187-376,
+                           main.cpp:15-16,44-54)
+
+Host arrays are numpy; device arrays are torch tensors on the GPU (torch is
+only the allocator and stream provider here). Every compute call goes
+through the HIP library; there is no CPU fallback: if the library or a GPU
+is missing, the call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "librbf_amd.so")
+
+OK, ERR_ARG, ERR_UNSUPPORTED, ERR_HIP, ERR_NOMEM, ERR_NODEVICE = range(6)
+OPS_DIFF, OPS_INTERP = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+STAT_STEPS_MASK, STAT_FALLBACK = 0x03, 0x04
+
+_lib = None
+
+
+class RbfError(RuntimeError):
+    def __init__(self, status, what):
+        self.status = status
+        super().__init__(f"{what}: {status_string(status)} (status {status})")
+
+
+class _BBox2(ctypes.Structure):
+    _fields_ = [("xmin", ctypes.c_double), ("ymin", ctypes.c_double),
+                ("xmax", ctypes.c_double), ("ymax", ctypes.c_double)]
+
+
+class _PlanDesc(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in
+                ("dim", "n", "degree", "phs", "op_set", "nrhs", "precision", "max_refine")]
+
+
+def lib():
+    """Load librbf_amd.so (built in-tree by gpu-assembly_amd/build.py)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with gpu-assembly_amd/build.py")
+    L = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    i32, i64, u64, dbl = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_double
+    sig = {
+        "rbfa_version": (ctypes.c_char_p, []),
+        "rbfa_status_string": (ctypes.c_char_p, [i32]),
+        "rbfa_morton_keys": (i32, [i32, P, P, P, i32, P]),
+        "rbfa_hilbert_keys": (i32, [i32, P, P, P, i32, P]),
+        "rbfa_morton_keys_3d": (i32, [i32, P, P, P, P, i32, P]),
+        "rbfa_order_by_keys": (i32, [i32, P, P]),
+        "rbfa_permute_f64": (i32, [i32, P, P]),
+        "rbfa_permute_i32": (i32, [i32, P, P]),
+        "rbfa_renumber_stencils": (i32, [i32, i32, P, P]),
+        "rbfa_make_row_ptr": (i32, [i32, i32, P]),
+        "rbfa_knn_stencils": (i32, [i32, i32, P, P, P, i32, P]),
+        "rbfa_gen_jittered_grid": (i32, [i32, i32, dbl, u64, P, P, P]),
+        "rbfa_plan_create": (i32, [P, P]),
+        "rbfa_plan_destroy": (i32, [P]),
+        "rbfa_plan_nops": (i32, [P]),
+        "rbfa_plan_system_size": (i32, [P]),
+        "rbfa_assemble": (i32, [P, i32, P, P, P, P, P, P, P, P, P, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def status_string(status):
+    return lib().rbfa_status_string(int(status)).decode()
+
+
+def _check(status, what):
+    if status != OK:
+        raise RbfError(status, what)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+# ---------------------------------------------------------------------------
+# Renumbering (rbf_reorder.h)
+# ---------------------------------------------------------------------------
+
+class Permutation:
+    """Gather-form permutation: map()[i] is the OLD index placed at NEW i."""
+
+    def __init__(self, new_to_old):
+        self._p = _i32(new_to_old)
+        n = self._p.size
+        if n and (self._p.min() < 0 or self._p.max() >= n or np.unique(self._p).size != n):
+            raise ValueError("not a permutation")
+        self._ip = np.empty_like(self._p)
+        self._ip[self._p] = np.arange(n, dtype=np.int32)
+
+    @classmethod
+    def identity(cls, n):
+        return cls(np.arange(n, dtype=np.int32))
+
+    @classmethod
+    def from_inverse(cls, old_to_new):
+        ip = _i32(old_to_new)
+        p = np.empty_like(ip)
+        p[ip] = np.arange(ip.size, dtype=np.int32)
+        return cls(p)
+
+    def size(self):
+        return self._p.size
+
+    def map(self):
+        return self._p
+
+    def inv(self):
+        return self._ip
+
+    def permute(self, a):
+        """a_new[i] = a_old[map()[i]] (returns a new array)."""
+        return np.asarray(a)[self._p]
+
+    def unpermute(self, a):
+        out = np.empty_like(np.asarray(a))
+        out[self._p] = a
+        return out
+
+    def then(self, q):
+        return Permutation(self._p[q.map()])
+
+    def inverse(self):
+        return Permutation(self._ip)
+
+
+def _keys(fn, x, y, ndiv, bbox):
+    x, y = _f64(x), _f64(y)
+    keys = np.empty(x.size, dtype=np.int64)
+    bb = None if bbox is None else ctypes.byref(_BBox2(*[float(v) for v in bbox]))
+    _check(fn(x.size, _ptr(x), _ptr(y), bb, int(ndiv), _ptr(keys)), fn.__name__)
+    return keys
+
+
+def morton_keys(x, y, ndiv=16, bbox=None):
+    """bbox = (xmin, ymin, xmax, ymax) or None for the points' own box."""
+    return _keys(lib().rbfa_morton_keys, x, y, ndiv, bbox)
+
+
+def hilbert_keys(x, y, ndiv=16, bbox=None):
+    return _keys(lib().rbfa_hilbert_keys, x, y, ndiv, bbox)
+
+
+def morton_keys_3d(x, y, z, ndiv=16, bbox=None):
+    x, y, z = _f64(x), _f64(y), _f64(z)
+    keys = np.empty(x.size, dtype=np.int64)
+    bb = None if bbox is None else _f64(bbox)
+    _check(lib().rbfa_morton_keys_3d(x.size, _ptr(x), _ptr(y), _ptr(z), _ptr(bb), int(ndiv),
+                                     _ptr(keys)), "rbfa_morton_keys_3d")
+    return keys
+
+
+def order_by_keys(keys):
+    keys = np.ascontiguousarray(keys, dtype=np.int64)
+    perm = np.empty(keys.size, dtype=np.int32)
+    _check(lib().rbfa_order_by_keys(keys.size, _ptr(keys), _ptr(perm)), "rbfa_order_by_keys")
+    return Permutation(perm)
+
+
+def morton_order(x, y, ndiv=16, bbox=None):
+    return order_by_keys(morton_keys(x, y, ndiv, bbox))
+
+
+def hilbert_order(x, y, ndiv=16, bbox=None):
+    return order_by_keys(hilbert_keys(x, y, ndiv, bbox))
+
+
+def morton_order_3d(x, y, z, ndiv=16, bbox=None):
+    return order_by_keys(morton_keys_3d(x, y, z, ndiv, bbox))
+
+
+def partition_last(pred):
+    """Stable partition: items with pred False first, then True
+    (partition_last, rbf_reorder.h:231-239)."""
+    pred = np.asarray(pred, bool)
+    return Permutation(np.argsort(pred, kind="stable").astype(np.int32))
+
+
+def boundary_last_by_flag(flag):
+    return partition_last(np.asarray(flag) != 0)
+
+
+def boundary_last_by_index(n, bnd):
+    mask = np.zeros(int(n), bool)
+    mask[np.asarray(bnd, np.int64)] = True
+    return partition_last(mask)
+
+
+def renumber_stencils(ja, k, p):
+    """A' = P A P^T on a fixed-width graph (returns a new array)."""
+    ja = _i32(ja).copy()
+    n = p.size()
+    _check(lib().rbfa_renumber_stencils(n, int(k), _ptr(p.map()), _ptr(ja)),
+           "rbfa_renumber_stencils")
+    return ja
+
+
+def make_row_ptr(n, k):
+    ia = np.empty(n + 1, dtype=np.int32)
+    _check(lib().rbfa_make_row_ptr(int(n), int(k), _ptr(ia)), "rbfa_make_row_ptr")
+    return ia
+
+
+def knn_stencils(k, x, y, z=None):
+    """ja of shape (n*k,): the k nearest of node s at ja[s*k], self first."""
+    x, y = _f64(x), _f64(y)
+    z = None if z is None else _f64(z)
+    ja = np.empty(x.size * int(k), dtype=np.int32)
+    _check(lib().rbfa_knn_stencils(x.size, 2 if z is None else 3, _ptr(x), _ptr(y), _ptr(z),
+                                   int(k), _ptr(ja)), "rbfa_knn_stencils")
+    return ja
+
+
+def jittered_grid(dim, m, jitter=0.25, seed=1):
+    """Jittered Cartesian nodes on the unit square (dim 2) or cube (dim 3)."""
+    n = m ** dim
+    x, y = np.empty(n), np.empty(n)
+    z = np.empty(n) if dim == 3 else None
+    _check(lib().rbfa_gen_jittered_grid(int(dim), int(m), float(jitter), int(seed), _ptr(x),
+                                        _ptr(y), _ptr(z)), "rbfa_gen_jittered_grid")
+    return (x, y) if dim == 2 else (x, y, z)
+
+
+class NodeSet:
+    """Node set with a renumbering chain (rbf_nodeset.h:46-235)."""
+
+    def __init__(self, x, y, flag=None, z=None):
+        self.x, self.y = _f64(x).copy(), _f64(y).copy()
+        self.z = None if z is None else _f64(z).copy()
+        self.flag = np.zeros(self.x.size, np.int32) if flag is None else _i32(flag).copy()
+        self._file_order = Permutation.identity(self.x.size)
+        self._rebuild_bnd()
+
+    def _rebuild_bnd(self):
+        self.bnd = np.nonzero(self.flag)[0].astype(np.int32)
+
+    def num_points(self):
+        return self.x.size
+
+    def num_boundary(self):
+        return self.bnd.size
+
+    def num_interior(self):
+        return self.x.size - self.bnd.size
+
+    def renumber(self, p):
+        self.x, self.y, self.flag = p.permute(self.x), p.permute(self.y), p.permute(self.flag)
+        if self.z is not None:
+            self.z = p.permute(self.z)
+        self._rebuild_bnd()
+        self._file_order = self._file_order.then(p)
+        return self
+
+    def file_order(self):
+        return self._file_order
+
+    def stencils(self, k):
+        if k < 1 or k > self.x.size:
+            raise ValueError(f"NodeSet.stencils: k={k} with {self.x.size} nodes")
+        return knn_stencils(k, self.x, self.y, self.z)
+
+
+# ---------------------------------------------------------------------------
+# Device assembly
+# ---------------------------------------------------------------------------
+
+class Plan:
+    """A compiled stencil shape: dim, stencil size n, degree P, PHS r^Q,
+    operator set ('diff' = Laplacian, d/dx, d/dy (, d/dz); 'interp' = nrhs
+    centres), precision ('fp64' or 'mixed')."""
+
+    def __init__(self, dim, n, degree, phs, op_set="diff", nrhs=0, precision="fp64",
+                 max_refine=3):
+        d = _PlanDesc(int(dim), int(n), int(degree), int(phs),
+                      OPS_DIFF if op_set == "diff" else OPS_INTERP, int(nrhs),
+                      PREC_FP64 if precision == "fp64" else PREC_MIXED, int(max_refine))
+        self._h = ctypes.c_void_p()
+        _check(lib().rbfa_plan_create(ctypes.byref(d), ctypes.byref(self._h)), "rbfa_plan_create")
+        self.dim, self.n, self.degree, self.phs = dim, n, degree, phs
+        self.op_set, self.precision = op_set, precision
+        self.nops = lib().rbfa_plan_nops(self._h)
+        self.nt = lib().rbfa_plan_system_size(self._h)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.rbfa_plan_destroy(h)
+            self._h = None
+
+    def assemble(self, nstencils, x, y, ja, w, z=None, centres=None, info=None, status=None,
+                 stream=None):
+        """Enqueue the assembly. x, y, z, ja, w[*], info, status are torch CUDA
+        tensors (float64 / int32 / uint8); centres = (xc, yc[, zc]) tensors."""
+        import torch
+
+        def dp(t):
+            return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+        if len(w) != self.nops:
+            raise ValueError(f"expected {self.nops} weight arrays")
+        nnz = int(nstencils) * self.n
+        for t in (x, y, ja, *w):
+            if not t.is_cuda or not t.is_contiguous():
+                raise ValueError("device tensors must be contiguous CUDA tensors")
+        if ja.dtype != torch.int32 or ja.numel() < nnz or any(
+                t.dtype != torch.float64 or t.numel() < nnz for t in w):
+            raise ValueError("ja must be int32 and w float64 with nstencils*n entries")
+        tab = (ctypes.c_void_p * self.nops)(*[t.data_ptr() for t in w])
+        cx = cy = cz = None
+        if centres is not None:
+            cx, cy = centres[0], centres[1]
+            cz = centres[2] if len(centres) > 2 else None
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        _check(lib().rbfa_assemble(self._h, int(nstencils), dp(x), dp(y), dp(z), dp(ja), tab,
+                                   dp(cx), dp(cy), dp(cz), dp(info), dp(status),
+                                   ctypes.c_void_p(stream)), "rbfa_assemble")

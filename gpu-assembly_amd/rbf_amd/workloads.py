@@ -1,0 +1,54 @@
+"""Seeded synthetic node sets and stencil graphs of the benchmarked shapes.
+
+Each builder returns a dict with the host arrays (x, y[, z]) in the final
+numbering, the stencil graph ja (int32, n per node, self first), and the
+basis (dim, n, P, Q). All of it comes from the host node layer of
+librbf_amd (jittered grid, curve renumbering, kNN).
+"""
+import time
+
+import numpy as np
+
+from . import hilbert_order, jittered_grid, knn_stencils, morton_order_3d
+
+SHAPES = {
+    # name: (dim, points per axis, n, P, Q, renumbering)
+    "square_4k_n21": (2, 64, 21, 2, 3, None),
+    "square_1m_n35": (2, 1024, 35, 3, 5, "hilbert"),
+    "square_1m_n56": (2, 1024, 56, 4, 7, "hilbert"),
+    "cube_2m_n80": (3, 128, 80, 3, 3, "morton"),
+}
+
+
+def build(name=None, dim=2, m=64, n=21, P=2, Q=3, order=None, seed=1, jitter=0.25):
+    if name is not None:
+        dim, m, n, P, Q, order = SHAPES[name]
+    t0 = time.perf_counter()
+    pts = jittered_grid(dim, m, jitter, seed)
+    t1 = time.perf_counter()
+    if order == "hilbert":
+        p = hilbert_order(pts[0], pts[1], 16)
+        pts = tuple(p.permute(a) for a in pts)
+    elif order == "morton":
+        p = morton_order_3d(*pts, ndiv=16)
+        pts = tuple(p.permute(a) for a in pts)
+    t2 = time.perf_counter()
+    ja = knn_stencils(n, *pts)
+    t3 = time.perf_counter()
+    out = {"dim": dim, "n": n, "P": P, "Q": Q, "x": pts[0], "y": pts[1],
+           "z": pts[2] if dim == 3 else None, "ja": ja, "nstencils": pts[0].size,
+           "host_s": {"nodes": t1 - t0, "renumber": t2 - t1, "knn": t3 - t2}}
+    return out
+
+
+def flops_per_stencil(nt, r):
+    """LAPACK operation counts: getrf(nt) + getrs(nt, r)."""
+    getrf = 2 * nt ** 3 / 3 - nt ** 2 / 2 - nt / 6
+    getrs = r * (2 * nt ** 2 - nt)
+    return getrf + getrs
+
+
+def bytes_per_stencil(n, r, dim=2):
+    """Algorithmic HBM bytes: ja (4n), the node's own coordinates (8 dim),
+    the weights (8 r n) and the solver status (4)."""
+    return 4 * n + 8 * dim + 8 * r * n + 4

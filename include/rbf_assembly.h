@@ -1,0 +1,190 @@
+/*
+ * rbf_assembly.h -- C ABI of the MI355X (gfx950) RBF-FD operator-assembly
+ * library (librbf_amd.so).
+ *
+ * Two layers sit behind this header:
+ *
+ *   1. The host node layer that produces the CSR structure: curve keys,
+ *      stable renumbering and k-nearest-neighbour stencils. Plain host
+ *      memory, OpenMP inside.
+ *   2. Batched operator assembly on the GPU: per stencil, gather, build the
+ *      polyharmonic-spline (PHS) + polynomial saddle-point system, factor it
+ *      with partial-pivoting LU and scatter the weights into CSR value
+ *      arrays that share the stencil column indices ja. All pointers given
+ *      to the assembly entry points are DEVICE pointers unless stated.
+ *
+ * Conventions (mirroring the reference's bind(c) layer):
+ *   - every entry point returns an int32 status, RBFA_OK == 0; where the
+ *     reference host library prints to stderr and calls exit(1)
+ *     (rbf_spatial.cpp:38-41, rbf_io.h:49-52), these return a code;
+ *   - scalars by value, arrays as plain pointers, int32 indices, int64
+ *     keys, double coordinates; 0-based indices throughout;
+ *   - per-stencil solver status info[s]: 0 on success, k > 0 when the k-th
+ *     pivot (1-based) was exactly zero, as the reference's per-block
+ *     solver status (rbf_operators.h:315,365); the host policy "count the
+ *     nonzero entries and fail" (main.cpp:48-54) is the caller's.
+ *
+ * Reference interfaces each entry point replaces are cited beside it as
+ * file:line of ivan-pi/gpu-assembly.
+ */
+#ifndef RBF_ASSEMBLY_H
+#define RBF_ASSEMBLY_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RBFA_API __attribute__((visibility("default")))
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+    RBFA_OK = 0,
+    RBFA_ERR_ARG = 1,         /* invalid argument (size, null pointer, range) */
+    RBFA_ERR_UNSUPPORTED = 2, /* no kernel instantiated for this shape       */
+    RBFA_ERR_HIP = 3,         /* a HIP runtime call failed                   */
+    RBFA_ERR_NOMEM = 4,       /* host or device allocation failed            */
+    RBFA_ERR_NODEVICE = 5     /* no usable gfx950 device                     */
+};
+
+/* ---- operator sets and precision modes --------------------------------- */
+enum {
+    /* Laplacian, d/dx, d/dy (and d/dz in 3-D), in this order: weights
+     * written to w[0], w[1], w[2] (, w[3]). The Laplacian entry point of
+     * the reference's pipeline sketch (main.cpp:15-16,44-45). */
+    RBFA_OPS_DIFF = 0,
+    /* Interpolation to nrhs centres given in the stencil-local frame
+     * (offsets from the stencil's own node), as assemble_interp_weights
+     * (rbf_operators.h:320-376, fill_rhs :158-185). */
+    RBFA_OPS_INTERP = 1
+};
+
+enum {
+    RBFA_PREC_FP64 = 0,  /* plain FP64 LU and solve                         */
+    RBFA_PREC_MIXED = 1  /* FP32 LU + FP64 iterative refinement, FP64 fallback */
+};
+
+/* Bits of the optional per-stencil status byte written by the mixed form. */
+enum {
+    RBFA_STAT_STEPS_MASK = 0x03, /* refinement steps taken (0..3)            */
+    RBFA_STAT_FALLBACK = 0x04    /* did not converge: solved again in FP64   */
+};
+
+/* Layout-compatible with rbf::BBox2<double> (rbf_reorder.h:30-40). */
+typedef struct rbfa_bbox2 {
+    double xmin, ymin, xmax, ymax;
+} rbfa_bbox2;
+
+RBFA_API const char* rbfa_version(void);
+RBFA_API const char* rbfa_status_string(int32_t status);
+
+/* ======================================================================
+ * Host node layer (all pointers are host pointers)
+ * ====================================================================== */
+
+/* Morton / Hilbert keys of n points inside bbox, ndiv quadtree levels
+ * (1..31), 2 bits per level. bbox == NULL: the bounding box of the points.
+ * Replaces rbf_morton_keys / rbf_hilbert_keys (rbf_ordering.F90:31-67,
+ * zidx :82-141, hidx :144-226) behind morton_keys / hilbert_keys
+ * (rbf_reorder.h:198-210). */
+RBFA_API int32_t rbfa_morton_keys(int32_t n, const double* x, const double* y,
+                                  const rbfa_bbox2* bbox, int32_t ndiv, int64_t* keys);
+RBFA_API int32_t rbfa_hilbert_keys(int32_t n, const double* x, const double* y,
+                                   const rbfa_bbox2* bbox, int32_t ndiv, int64_t* keys);
+
+/* 3-D Morton keys, 3 bits per level, ndiv 1..21; bbox = {xmin, ymin, zmin,
+ * xmax, ymax, zmax} or NULL. The reference's curves are 2-D only
+ * (rbf_ordering.F90:8); this extends zidx's midpoint rule to an octree. */
+RBFA_API int32_t rbfa_morton_keys_3d(int32_t n, const double* x, const double* y,
+                                     const double* z, const double* bbox6, int32_t ndiv,
+                                     int64_t* keys);
+
+/* Stable order of the keys: perm[i] is the old index of the item placed
+ * at new position i (gather form). Replaces detail::order_by_keys
+ * (rbf_reorder.h:183-189) and hence morton_order / hilbert_order
+ * (:214-229). */
+RBFA_API int32_t rbfa_order_by_keys(int32_t n, const int64_t* keys, int32_t* perm);
+
+/* a_new[i] = a_old[perm[i]], in place. Permutation::permute
+ * (rbf_reorder.h:108-115). */
+RBFA_API int32_t rbfa_permute_f64(int32_t n, const int32_t* perm, double* a);
+RBFA_API int32_t rbfa_permute_i32(int32_t n, const int32_t* perm, int32_t* a);
+
+/* Fixed-width stencil graph renumbered in place, A' = P A P^T
+ * (renumber_stencils, rbf_reorder.h:278-288). */
+RBFA_API int32_t rbfa_renumber_stencils(int32_t n, int32_t k, const int32_t* perm,
+                                        int32_t* ja);
+
+/* Row pointer {0, k, 2k, ...} of length n+1 (make_row_ptr,
+ * rbf_reorder.h:316-322). */
+RBFA_API int32_t rbfa_make_row_ptr(int32_t n, int32_t k, int32_t* ia);
+
+/* k nearest neighbours of every node, ja[s*k + j], sorted by squared
+ * distance (dx*dx + dy*dy (+ dz*dz), query minus point), ties by the
+ * lower index, so ja[s*k] == s for distinct points. dim is 2 or 3 (z
+ * ignored for 2). Replaces NodeSet::stencils (rbf_nodeset.h:155-181,
+ * nanoflann knnSearch) and KdTree::knn_stencils (rbf_spatial.cpp:251-260). */
+RBFA_API int32_t rbfa_knn_stencils(int32_t n, int32_t dim, const double* x, const double* y,
+                                   const double* z, int32_t k, int32_t* ja);
+
+/* Synthetic node sets (seeded xorshift64, the generator family of the
+ * reference's test_spatial.cpp:39-47).
+ *
+ * Jittered Cartesian grid on the unit square / cube: m points per axis at
+ * ((i + 1/2) h) + jitter * h * (2u - 1), h = 1/m, row-major with x
+ * fastest. dim 2 writes m^2 points to x, y; dim 3 writes m^3 to x, y, z. */
+RBFA_API int32_t rbfa_gen_jittered_grid(int32_t dim, int32_t m, double jitter, uint64_t seed,
+                                        double* x, double* y, double* z);
+
+/* ======================================================================
+ * Device assembly
+ * ====================================================================== */
+
+typedef struct rbfa_plan_desc {
+    int32_t dim;        /* 2 or 3                                            */
+    int32_t n;          /* stencil size (neighbours per node, self first)   */
+    int32_t degree;     /* polynomial augmentation degree P (0..4)          */
+    int32_t phs;        /* PHS exponent Q: phi(r) = r^Q, odd, 3..7          */
+    int32_t op_set;     /* RBFA_OPS_DIFF or RBFA_OPS_INTERP                 */
+    int32_t nrhs;       /* RBFA_OPS_INTERP: number of centres; else ignored */
+    int32_t precision;  /* RBFA_PREC_FP64 or RBFA_PREC_MIXED                */
+    int32_t max_refine; /* mixed form: refinement steps before fallback, 1..3 */
+} rbfa_plan_desc;
+
+typedef struct rbfa_plan rbfa_plan;
+
+/* The plan replaces the compile-time interp_config<N,NRHS,P,Q,ARCH,T>
+ * (rbf_operators.h:187-290): it picks the kernel instantiation and launch
+ * shape for the gfx950 device current at creation time. */
+RBFA_API int32_t rbfa_plan_create(const rbfa_plan_desc* desc, rbfa_plan** plan);
+RBFA_API int32_t rbfa_plan_destroy(rbfa_plan* plan);
+/* Number of operators (value arrays) the plan writes. */
+RBFA_API int32_t rbfa_plan_nops(const rbfa_plan* plan);
+/* nt = n + npoly, the order of each saddle-point system. */
+RBFA_API int32_t rbfa_plan_system_size(const rbfa_plan* plan);
+
+/* Assemble nstencils stencils on `stream` (a hipStream_t; NULL = default).
+ *
+ *   x, y, z   node coordinates (device; z only for dim 3), indexed by ja
+ *   ja        nstencils*n column indices, stencil s at ja[s*n], ja[s*n] == s
+ *   w         HOST array of nops DEVICE pointers, each to nstencils*n
+ *             doubles laid out like ja: w[op][s*n + k] is the weight of
+ *             node ja[s*n + k] in operator op of row s
+ *   xc,yc,zc  RBFA_OPS_INTERP only: nrhs centres (device), local frame
+ *   info      nstencils int32 solver statuses (device), may be NULL
+ *   status    nstencils status bytes (device, RBFA_STAT_*), may be NULL
+ *
+ * Replaces the launch of assemble_interp_weights<Config>
+ * (rbf_operators.h:317-328) and the sketched assemble_laplace_weights
+ * (main.cpp:44-45). Asynchronous: returns once the work is enqueued. */
+RBFA_API int32_t rbfa_assemble(const rbfa_plan* plan, int32_t nstencils, const double* x,
+                               const double* y, const double* z, const int32_t* ja,
+                               double* const* w, const double* xc, const double* yc,
+                               const double* zc, int32_t* info, uint8_t* status, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RBF_ASSEMBLY_H */

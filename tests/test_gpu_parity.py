@@ -1,0 +1,110 @@
+"""GPU parity: the HIP assembly (through the C ABI) against the CPU FP64
+oracle on the same seeded nodes and stencils.
+
+Bars: FP64 form, max over stencils and operators of
+max_k |w - w_ref| / max_k |w_ref| <= 1e-10; mixed form <= 1e-8.
+"""
+import numpy as np
+import pytest
+
+import rbf_amd as R
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+MIXED_TOL = 1e-8
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+
+
+def _run(plan, x, y, ja, z=None, centres=None):
+    import torch
+    nst = ja.size // plan.n
+    w = [torch.full((ja.size,), np.nan, dtype=torch.float64, device="cuda:0")
+         for _ in range(plan.nops)]
+    info = torch.full((nst,), -7, dtype=torch.int32, device="cuda:0")
+    status = torch.full((nst,), 255, dtype=torch.uint8, device="cuda:0")
+    cen = None if centres is None else tuple(_dev(c) for c in centres)
+    plan.assemble(nst, _dev(x), _dev(y), _dev(ja), w, z=None if z is None else _dev(z),
+                  centres=cen, info=info, status=status)
+    torch.cuda.synchronize()
+    return [t.cpu().numpy() for t in w], info.cpu().numpy(), status.cpu().numpy()
+
+
+def _relerr(w, wref, n):
+    e = [np.abs(a.reshape(-1, n) - b.reshape(-1, n)).max(1) /
+         np.abs(b.reshape(-1, n)).max(1) for a, b in zip(w, wref)]
+    return np.stack(e, 1)
+
+
+SHAPES_2D = [(64, 21, 2, 3), (64, 35, 3, 5), (48, 13, 2, 3), (40, 35, 3, 3), (40, 21, 2, 5),
+             (40, 13, 1, 3)]
+
+
+@pytest.mark.parametrize("m,n,P,Q", SHAPES_2D)
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_diff_ops_match_oracle(m, n, P, Q, prec):
+    try:
+        plan = R.Plan(2, n, P, Q, "diff", precision=prec)
+    except R.RbfError as e:
+        if e.status == R.ERR_UNSUPPORTED:
+            pytest.skip("shape not shipped in this precision")
+        raise
+    x, y = R.jittered_grid(2, m, 0.25, seed=7)
+    ja = R.knn_stencils(n, x, y)
+    w, info, status = _run(plan, x, y, ja)
+    wref, iref = O.assemble(2, n, P, Q, x, y, ja)
+    assert (info == 0).all() and (iref == 0).all()
+    e = _relerr(w, wref, n)
+    tol = FP64_TOL if prec == "fp64" else MIXED_TOL
+    print(f"{prec} m={m} n={n} P={P} Q={Q}: max rel err {e.max():.3e}")
+    assert e.max() <= tol
+    if prec == "fp64":
+        assert (status == 0).all()
+    else:
+        assert ((status & 0x3) <= 3).all()
+
+
+def test_interp_d2q9_matches_oracle():
+    """Interpolation to the 9 D2Q9 departure points of each node, the
+    reference demo's NRHS = 9 (main.f90:16,55: centres 0.5*(x, y))."""
+    n, P, Q = 9, 2, 3
+    plan = R.Plan(2, n, P, Q, "interp", nrhs=9)
+    x, y = R.jittered_grid(2, 32, 0.25, seed=3)
+    ja = R.knn_stencils(n, x, y)
+    h = 1.0 / 32
+    cx = 0.5 * h * np.array([0, 1, 0, -1, 0, 1, -1, -1, 1], float)
+    cy = 0.5 * h * np.array([0, 0, 1, 0, -1, 1, 1, -1, -1], float)
+    w, info, _ = _run(plan, x, y, ja, centres=(cx, cy))
+    wref, iref = O.assemble(2, n, P, Q, x, y, ja, op_set=1, centres=(cx, cy))
+    assert (info == 0).all()
+    e = _relerr(w, wref, n)
+    assert e.max() <= FP64_TOL
+    # interpolation reproduces the quadratics exactly
+    J = ja.reshape(-1, n)
+    for c in range(9):
+        f = x ** 2 - x * y + 0.5 * y
+        v = (w[c].reshape(-1, n) * f[J]).sum(1)
+        ex = (x + cx[c]) ** 2 - (x + cx[c]) * (y + cy[c]) + 0.5 * (y + cy[c])
+        assert np.abs(v - ex).max() < 1e-11
+
+
+def test_singular_stencil_reports_info():
+    """Exactly duplicated nodes give two identical PHS rows: info > 0, as
+    the reference's per-stencil status (main.cpp:48-54)."""
+    n, P, Q = 21, 2, 3
+    x, y = R.jittered_grid(2, 16, 0.25, seed=5)
+    x[10], y[10] = x[11], y[11]
+    ja = R.knn_stencils(n, x, y)
+    plan = R.Plan(2, n, P, Q, "diff")
+    w, info, _ = _run(plan, x, y, ja)
+    wref, iref = O.assemble(2, n, P, Q, x, y, ja)
+    assert (info != 0).sum() > 0
+    np.testing.assert_array_equal(info != 0, iref != 0)
+    ok = iref == 0
+    assert _relerr([a.reshape(-1, n)[ok] for a in w], [b.reshape(-1, n)[ok] for b in wref],
+                   n).max() <= FP64_TOL
