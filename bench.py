@@ -98,6 +98,7 @@ def main():
                     help="stencils in the CPU-baseline / oracle-parity sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity")
     ap.add_argument("--form", choices=["both", "fp64", "mixed"], default="both")
+    ap.add_argument("--no-tune", action="store_true", help="skip the variant autotuner")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -131,7 +132,13 @@ def main():
         def step():
             plan.assemble(nst, x, y, ja, w, info=info, status=status)
 
-        log(f"[{form}] warmup {args.warmup}, timing {args.steps} steps")
+        tune_ms = None
+        if not args.no_tune:
+            tune_ms = plan.autotune(nst, x, y, ja, w, info=info, status=status, reps=2)
+            log(f"[{form}] autotune: " + ", ".join(
+                f"{plan.variant_name(v)}={t:.3f}ms" for v, t in enumerate(tune_ms)))
+        log(f"[{form}] variant {plan.variant_name()}; warmup {args.warmup}, "
+            f"timing {args.steps} steps")
         ms, wall = hip_timed(step, args.steps, args.warmup)
         bad = int((info != 0).sum().item())
         fb = int(((status & R.STAT_FALLBACK) != 0).sum().item()) if form == "mixed" else 0
@@ -140,6 +147,9 @@ def main():
             st = (status & R.STAT_STEPS_MASK).to(torch.int64)
             steps_hist = [int((st == k).sum().item()) for k in range(4)]
         res[form] = {"ms": ms, "wall_s": wall, "singular": bad, "fallback": fb,
+                     "variant": plan.variant_name(),
+                     "autotune_ms": None if tune_ms is None else
+                     {plan.variant_name(v): round(t, 4) for v, t in enumerate(tune_ms)},
                      "refine_steps_hist": steps_hist, "nt": nt, "r": r,
                      "w": [t.cpu().numpy() for t in w]}
         log(f"[{form}] {ms:.3f} ms/step, {nst / (ms * 1e-3) / 1e6:.1f} Mstencils/s, "
@@ -162,7 +172,8 @@ def main():
                        "gflops": flop * sps / 1e9,
                        "frac_fp64_peak": flop * sps / 1e12 / PEAK_FP64_TFLOPS,
                        "scatter_gbs": byt * sps / 1e9, "singular": d["singular"],
-                       "fallback_fraction": d["fallback"] / nst}
+                       "fallback_fraction": d["fallback"] / nst, "variant": d["variant"],
+                       "autotune_ms": d["autotune_ms"]}
         if d["refine_steps_hist"] is not None:
             extra[form]["refine_steps_hist"] = d["refine_steps_hist"]
     if "fp64" in res and "mixed" in res:

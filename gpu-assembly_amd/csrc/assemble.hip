@@ -37,81 +37,142 @@ struct LaunchArgs {
 // MI355X dispatches consecutive workgroups round-robin over its 8 XCDs,
 // each with its own L2. Renumber so each XCD works on one contiguous range
 // of (space-filling-curve ordered) stencils and its L2 sees neighbouring
-// coordinates: block b runs stencil (b % 8) * chunk + b / 8.
-__device__ __forceinline__ int xcd_stencil(int nstencils) {
+// coordinates: block b runs block-range (b % 8) * chunk + b / 8.
+__device__ __forceinline__ int xcd_block(int nblocks) {
     const int b = static_cast<int>(blockIdx.x);
-    const int chunk = (nstencils + 7) / 8;
+    const int chunk = (nblocks + 7) / 8;
     return (b & 7) * chunk + (b >> 3);
 }
-inline unsigned xcd_grid(int nstencils) { return 8u * static_cast<unsigned>((nstencils + 7) / 8); }
+inline unsigned xcd_grid(int nblocks) { return 8u * static_cast<unsigned>((nblocks + 7) / 8); }
 
-template <class E>
-__global__ void __launch_bounds__(64)
+// WPB waves per workgroup, one stencil per wave, each wave with its own
+// LDS region.
+template <class E, int WPB>
+__global__ void __launch_bounds__(64 * WPB)
 k_assemble_fp64(int nstencils, const double* __restrict__ x, const double* __restrict__ y,
                 const double* __restrict__ z, const int* __restrict__ ja,
                 detail::ValueOut<double, E::NC - E::NT> out, const double* __restrict__ cx,
                 const double* __restrict__ cy, const double* __restrict__ cz,
                 int* __restrict__ info, uint8_t* __restrict__ status) {
-    __shared__ double lds[E::lds_doubles];
-    const int s = xcd_stencil(nstencils);
-    if (s >= nstencils) return;
-    E::run(s, x, y, z, ja, out, cx, cy, cz, lds, info);
+    __shared__ __align__(16) double lds[WPB * E::lds_doubles];
+    const int nblocks = (nstencils + WPB - 1) / WPB;
+    const int b = xcd_block(nblocks);
+    const int w = static_cast<int>(threadIdx.x >> 6);
+    const int s = b * WPB + w;
+    if (b >= nblocks || s >= nstencils) return;
+    E::run(s, x, y, z, ja, out, cx, cy, cz, lds + w * E::lds_doubles, info);
     if (status && detail::lane_id() == 0) status[s] = 0;
 }
 
-template <class E>
+template <class E, int WPB>
 hipError_t launch_fp64(const LaunchArgs& a) {
     constexpr int R = E::NC - E::NT;
     detail::ValueOut<double, R> out;
     for (int c = 0; c < R; ++c) out.p[c] = a.w[c];
-    hipLaunchKernelGGL(k_assemble_fp64<E>, dim3(xcd_grid(a.nstencils)), dim3(64), 0, a.stream,
-                       a.nstencils, a.x, a.y, a.z, a.ja, out, a.cx, a.cy, a.cz, a.info, a.status);
+    const int nblocks = (a.nstencils + WPB - 1) / WPB;
+    hipLaunchKernelGGL((k_assemble_fp64<E, WPB>), dim3(xcd_grid(nblocks)), dim3(64 * WPB), 0,
+                       a.stream, a.nstencils, a.x, a.y, a.z, a.ja, out, a.cx, a.cy, a.cz, a.info,
+                       a.status);
     return hipGetLastError();
 }
 
-template <class E>
-__global__ void __launch_bounds__(64)
+template <class E, int WPB>
+__global__ void __launch_bounds__(64 * WPB)
 k_assemble_mixed(int nstencils, const double* __restrict__ x, const double* __restrict__ y,
                  const double* __restrict__ z, const int* __restrict__ ja,
                  detail::ValueOut<double, E::R> out, int* __restrict__ info,
                  uint8_t* __restrict__ status, int max_refine) {
-    __shared__ double lds[E::lds_doubles];
-    const int s = xcd_stencil(nstencils);
-    if (s >= nstencils) return;
-    E::run(s, x, y, z, ja, out, lds, info, status, max_refine);
+    __shared__ __align__(16) double lds[WPB * E::lds_doubles];
+    const int nblocks = (nstencils + WPB - 1) / WPB;
+    const int b = xcd_block(nblocks);
+    const int w = static_cast<int>(threadIdx.x >> 6);
+    const int s = b * WPB + w;
+    if (b >= nblocks || s >= nstencils) return;
+    E::run(s, x, y, z, ja, out, lds + w * E::lds_doubles, info, status, max_refine);
 }
 
-template <class E>
+template <class E, int WPB>
 hipError_t launch_mixed(const LaunchArgs& a) {
     detail::ValueOut<double, E::R> out;
     for (int c = 0; c < E::R; ++c) out.p[c] = a.w[c];
-    hipLaunchKernelGGL(k_assemble_mixed<E>, dim3(xcd_grid(a.nstencils)), dim3(64), 0, a.stream,
-                       a.nstencils, a.x, a.y, a.z, a.ja, out, a.info, a.status, a.max_refine);
+    const int nblocks = (a.nstencils + WPB - 1) / WPB;
+    hipLaunchKernelGGL((k_assemble_mixed<E, WPB>), dim3(xcd_grid(nblocks)), dim3(64 * WPB), 0,
+                       a.stream, a.nstencils, a.x, a.y, a.z, a.ja, out, a.info, a.status,
+                       a.max_refine);
     return hipGetLastError();
 }
 
 using LaunchFn = hipError_t (*)(const LaunchArgs&);
 
+struct Variant {
+    const char* name;
+    LaunchFn fn;
+};
+
+constexpr int kVariants = 6;
+
+// The tuning space per shape: pivot-row broadcast (readlane / LDS / half
+// and half) x pivot search (shuffle / DPP+SALU) x waves per workgroup.
+template <template <class, int> class L, template <class> class Eng>
+struct VariantSet {
+    static constexpr Variant list[kVariants] = {
+        {"bc=readlane pv=shfl wpb=1", &L<Eng<detail::LuTune<0, 0>>, 1>::run},
+        {"bc=lds pv=shfl wpb=1", &L<Eng<detail::LuTune<1, 0>>, 1>::run},
+        {"bc=half pv=shfl wpb=1", &L<Eng<detail::LuTune<2, 0>>, 1>::run},
+        {"bc=lds pv=dpp wpb=1", &L<Eng<detail::LuTune<1, 1>>, 1>::run},
+        {"bc=half pv=dpp wpb=1", &L<Eng<detail::LuTune<2, 1>>, 1>::run},
+        {"bc=lds pv=shfl wpb=4", &L<Eng<detail::LuTune<1, 0>>, 4>::run},
+    };
+};
+
+template <class E, int WPB>
+struct Fp64L {
+    static hipError_t run(const LaunchArgs& a) { return launch_fp64<E, WPB>(a); }
+};
+template <class E, int WPB>
+struct MixL {
+    static hipError_t run(const LaunchArgs& a) { return launch_mixed<E, WPB>(a); }
+};
+
 struct Shape {
     int dim, n, P, Q, op_set, nrhs, precision;
     int nt, nops;
-    LaunchFn fn;
+    const Variant* variants;
+};
+
+template <int DIM, int N, int P, int Q>
+struct DiffEng {
+    template <class T>
+    using E = detail::WaveAssembly<DIM, N, P, Q, DIM + 1, false, T>;
+};
+template <int N, int P, int Q, int R>
+struct InterpEng {
+    template <class T>
+    using E = detail::WaveAssembly<2, N, P, Q, R, true, T>;
+};
+template <int DIM, int N, int P, int Q>
+struct MixEng {
+    template <class T>
+    using E = detail::WaveMixed<DIM, N, P, Q, T>;
 };
 
 template <int DIM, int N, int P, int Q>
 constexpr Shape diff64() {
     using E = detail::WaveAssembly<DIM, N, P, Q, DIM + 1, false>;
-    return {DIM, N, P, Q, RBFA_OPS_DIFF, 0, RBFA_PREC_FP64, E::NT, DIM + 1, &launch_fp64<E>};
+    return {DIM, N, P, Q, RBFA_OPS_DIFF, 0, RBFA_PREC_FP64, E::NT, DIM + 1,
+            VariantSet<Fp64L, DiffEng<DIM, N, P, Q>::template E>::list};
 }
 template <int N, int P, int Q, int R>
 constexpr Shape interp64() {
     using E = detail::WaveAssembly<2, N, P, Q, R, true>;
-    return {2, N, P, Q, RBFA_OPS_INTERP, R, RBFA_PREC_FP64, E::NT, R, &launch_fp64<E>};
+    return {2, N, P, Q, RBFA_OPS_INTERP, R, RBFA_PREC_FP64, E::NT, R,
+            VariantSet<Fp64L, InterpEng<N, P, Q, R>::template E>::list};
 }
 template <int DIM, int N, int P, int Q>
 constexpr Shape diffmix() {
     using E = detail::WaveMixed<DIM, N, P, Q>;
-    return {DIM, N, P, Q, RBFA_OPS_DIFF, 0, RBFA_PREC_MIXED, E::NT, DIM + 1, &launch_mixed<E>};
+    return {DIM, N, P, Q, RBFA_OPS_DIFF, 0, RBFA_PREC_MIXED, E::NT, DIM + 1,
+            VariantSet<MixL, MixEng<DIM, N, P, Q>::template E>::list};
 }
 
 // The shapes this library ships: the stencil families of the workloads it
@@ -130,6 +191,7 @@ const Shape kShapes[] = {
 struct rbfa_plan {
     rbfa_plan_desc desc;
     Shape shape;
+    int variant;
 };
 
 extern "C" {
@@ -156,7 +218,7 @@ RBFA_API int32_t rbfa_plan_create(const rbfa_plan_desc* desc, rbfa_plan** plan) 
             hipDeviceProp_t prop;
             if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return RBFA_ERR_NODEVICE;
             if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return RBFA_ERR_NODEVICE;
-            auto* p = new (std::nothrow) rbfa_plan{d, s};
+            auto* p = new (std::nothrow) rbfa_plan{d, s, 0};
             if (!p) return RBFA_ERR_NOMEM;
             *plan = p;
             return RBFA_OK;
@@ -201,7 +263,61 @@ RBFA_API int32_t rbfa_assemble(const rbfa_plan* plan, int32_t nstencils, const d
     a.status = status;
     a.max_refine = plan->desc.max_refine;
     a.stream = static_cast<hipStream_t>(stream);
-    return sh.fn(a) == hipSuccess ? RBFA_OK : RBFA_ERR_HIP;
+    return sh.variants[plan->variant].fn(a) == hipSuccess ? RBFA_OK : RBFA_ERR_HIP;
+}
+
+RBFA_API int32_t rbfa_plan_num_variants(const rbfa_plan* plan) { return plan ? kVariants : -1; }
+
+RBFA_API const char* rbfa_plan_variant_name(const rbfa_plan* plan, int32_t v) {
+    if (!plan || v < 0 || v >= kVariants) return nullptr;
+    return plan->shape.variants[v].name;
+}
+
+RBFA_API int32_t rbfa_plan_get_variant(const rbfa_plan* plan) { return plan ? plan->variant : -1; }
+
+RBFA_API int32_t rbfa_plan_set_variant(rbfa_plan* plan, int32_t v) {
+    if (!plan || v < 0 || v >= kVariants) return RBFA_ERR_ARG;
+    plan->variant = v;
+    return RBFA_OK;
+}
+
+RBFA_API int32_t rbfa_plan_autotune(rbfa_plan* plan, int32_t nstencils, const double* x,
+                                    const double* y, const double* z, const int32_t* ja,
+                                    double* const* w, const double* xc, const double* yc,
+                                    const double* zc, int32_t* info, uint8_t* status,
+                                    void* stream, int32_t reps, float* ms_out) {
+    if (!plan || reps < 1) return RBFA_ERR_ARG;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return RBFA_ERR_HIP;
+    int best = plan->variant;
+    float best_ms = 0.f;
+    int32_t rc = RBFA_OK;
+    for (int v = 0; v < kVariants && rc == RBFA_OK; ++v) {
+        plan->variant = v;
+        rc = rbfa_assemble(plan, nstencils, x, y, z, ja, w, xc, yc, zc, info, status, stream);
+        if (rc != RBFA_OK) break;
+        if (hipEventRecord(e0, st) != hipSuccess) { rc = RBFA_ERR_HIP; break; }
+        for (int r = 0; r < reps && rc == RBFA_OK; ++r)
+            rc = rbfa_assemble(plan, nstencils, x, y, z, ja, w, xc, yc, zc, info, status, stream);
+        if (rc != RBFA_OK) break;
+        float ms = 0.f;
+        if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = RBFA_ERR_HIP;
+            break;
+        }
+        ms /= reps;
+        if (ms_out) ms_out[v] = ms;
+        if (v == 0 || ms < best_ms) {
+            best = v;
+            best_ms = ms;
+        }
+    }
+    plan->variant = best;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
 }
 
 }  // extern "C"

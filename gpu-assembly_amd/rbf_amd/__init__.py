@@ -79,6 +79,11 @@ def lib():
         "rbfa_plan_nops": (i32, [P]),
         "rbfa_plan_system_size": (i32, [P]),
         "rbfa_assemble": (i32, [P, i32, P, P, P, P, P, P, P, P, P, P, P]),
+        "rbfa_plan_num_variants": (i32, [P]),
+        "rbfa_plan_variant_name": (ctypes.c_char_p, [P, i32]),
+        "rbfa_plan_get_variant": (i32, [P]),
+        "rbfa_plan_set_variant": (i32, [P, i32]),
+        "rbfa_plan_autotune": (i32, [P, i32, P, P, P, P, P, P, P, P, P, P, P, i32, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -323,10 +328,39 @@ class Plan:
             _lib.rbfa_plan_destroy(h)
             self._h = None
 
+    @property
+    def num_variants(self):
+        return lib().rbfa_plan_num_variants(self._h)
+
+    def variant_name(self, v=None):
+        v = self.variant if v is None else v
+        return lib().rbfa_plan_variant_name(self._h, int(v)).decode()
+
+    @property
+    def variant(self):
+        return lib().rbfa_plan_get_variant(self._h)
+
+    @variant.setter
+    def variant(self, v):
+        _check(lib().rbfa_plan_set_variant(self._h, int(v)), "rbfa_plan_set_variant")
+
+    def autotune(self, nstencils, x, y, ja, w, z=None, centres=None, info=None, status=None,
+                 reps=3, stream=None):
+        """Time every kernel variant on these inputs, keep the fastest; returns
+        the per-variant milliseconds."""
+        args = self._args(nstencils, x, y, ja, w, z, centres, info, status, stream)
+        ms = (ctypes.c_float * self.num_variants)()
+        _check(lib().rbfa_plan_autotune(self._h, *args, int(reps), ms), "rbfa_plan_autotune")
+        return list(ms)
+
     def assemble(self, nstencils, x, y, ja, w, z=None, centres=None, info=None, status=None,
                  stream=None):
         """Enqueue the assembly. x, y, z, ja, w[*], info, status are torch CUDA
         tensors (float64 / int32 / uint8); centres = (xc, yc[, zc]) tensors."""
+        args = self._args(nstencils, x, y, ja, w, z, centres, info, status, stream)
+        _check(lib().rbfa_assemble(self._h, *args), "rbfa_assemble")
+
+    def _args(self, nstencils, x, y, ja, w, z, centres, info, status, stream):
         import torch
 
         def dp(t):
@@ -348,6 +382,6 @@ class Plan:
             cz = centres[2] if len(centres) > 2 else None
         if stream is None:
             stream = torch.cuda.current_stream().cuda_stream
-        _check(lib().rbfa_assemble(self._h, int(nstencils), dp(x), dp(y), dp(z), dp(ja), tab,
-                                   dp(cx), dp(cy), dp(cz), dp(info), dp(status),
-                                   ctypes.c_void_p(stream)), "rbfa_assemble")
+        self._keep = tab
+        return (int(nstencils), dp(x), dp(y), dp(z), dp(ja), tab, dp(cx), dp(cy), dp(cz),
+                dp(info), dp(status), ctypes.c_void_p(stream))

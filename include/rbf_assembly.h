@@ -183,6 +183,26 @@ RBFA_API int32_t rbfa_assemble(const rbfa_plan* plan, int32_t nstencils, const d
                                double* const* w, const double* xc, const double* yc,
                                const double* zc, int32_t* info, uint8_t* status, void* stream);
 
+/* ---- kernel variants and the autotuner --------------------------------
+ * Each shape ships several kernel variants (pivot-row broadcast through
+ * SGPRs or LDS, pivot search by shuffles or DPP, waves per workgroup). All
+ * give identical results; they differ in speed. These have no counterpart
+ * in the reference, where cuSolverDx fixes block_dim at compile time
+ * (rbf_operators.h:280-281). */
+RBFA_API int32_t rbfa_plan_num_variants(const rbfa_plan* plan);
+RBFA_API const char* rbfa_plan_variant_name(const rbfa_plan* plan, int32_t variant);
+RBFA_API int32_t rbfa_plan_get_variant(const rbfa_plan* plan);
+RBFA_API int32_t rbfa_plan_set_variant(rbfa_plan* plan, int32_t variant);
+/* Time every variant on the given inputs (reps launches each, HIP events on
+ * `stream`, synchronising it), keep the fastest in the plan, and write the
+ * per-variant milliseconds to ms_out[num_variants] if non-NULL. The
+ * outputs w/info/status are overwritten. */
+RBFA_API int32_t rbfa_plan_autotune(rbfa_plan* plan, int32_t nstencils, const double* x,
+                                    const double* y, const double* z, const int32_t* ja,
+                                    double* const* w, const double* xc, const double* yc,
+                                    const double* zc, int32_t* info, uint8_t* status,
+                                    void* stream, int32_t reps, float* ms_out);
+
 #ifdef __cplusplus
 }
 #endif

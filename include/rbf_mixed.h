@@ -24,9 +24,10 @@
 namespace rbf_operators {
 namespace detail {
 
-template <int DIM, int N, int P, int Q>
+template <int DIM, int N, int P, int Q, class Tune = DefaultTune>
 struct WaveMixed {
-    using Base = WaveAssembly<DIM, N, P, Q, DIM + 1, false>;
+    using Base = WaveAssembly<DIM, N, P, Q, DIM + 1, false, Tune>;
+    using tune = Tune;
     static constexpr int M = Base::M, NT = Base::NT, NC = Base::NC, R = DIM + 1;
     // Base scratch + the iterate x, [R][64] doubles, for the residual.
     static constexpr int lds_doubles = Base::lds_doubles + R * 64;
@@ -34,47 +35,20 @@ struct WaveMixed {
 
     // FP32 LU of a32 in place: multipliers stay below the pivots. Returns
     // nonzero if a pivot was exactly zero (or every candidate NaN).
-    __device__ static int lu32(float (&a)[NC], int (&perm)[NT], int& mystep, float& myrcp) {
-        const int lane = lane_id();
-        bool done = false;
-        int bad = 0;
-        mystep = 64;
-        myrcp = 0.f;
+    __device__ __forceinline__ static int lu32(float (&a)[NC], int (&perm)[NT], int& mystep, float& myrcp,
+                               float* pb) {
+        ElimState<NT, float> st;
+        elim_from<0, NT, NC, Tune, true>(a, st, pb, lane_id());
 #pragma unroll
-        for (int k = 0; k < NT; ++k) {
-            const bool active = lane < NT && !done;
-            const float ak = a[k];
-            const float cand = active ? fabsf(ak) : -1.f;
-            const float mx = wave_max(cand);
-            const unsigned long long hit = __ballot(active && cand == mx);
-            int p = hit ? __ffsll(hit) - 1 : __ffsll(__ballot(active)) - 1;
-            if (!hit) bad = 1;
-            p = __builtin_amdgcn_readfirstlane(p);
-            const float piv = bcast(ak, p);
-            if (piv == 0.f) bad = 1;
-            const float rcp = 1.f / piv;
-            const bool is_piv = lane == p;
-            const bool elim = active && !is_piv;
-            const float l = elim ? ak * rcp : 0.f;
-            a[k] = elim ? l : ak;
-            if (is_piv) {
-                done = true;
-                mystep = k;
-                myrcp = rcp;
-            }
-            perm[k] = p;
-#pragma unroll
-            for (int j = k + 1; j < NC; ++j) {
-                const float u = bcast(a[j], p);
-                a[j] = fmaf(-l, u, a[j]);
-            }
-        }
-        return bad;
+        for (int k = 0; k < NT; ++k) perm[k] = st.perm[k];
+        mystep = st.mystep;
+        myrcp = st.myrcp;
+        return st.info;
     }
 
     // Solve L U d = r for the R columns held one equation per lane;
     // on return lane i holds the component of unknown mystep.
-    __device__ static void solve32(const float (&a)[NC], const int (&perm)[NT], int mystep,
+    __device__ __forceinline__ static void solve32(const float (&a)[NC], const int (&perm)[NT], int mystep,
                                    float myrcp, float (&r)[R]) {
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
@@ -95,7 +69,7 @@ struct WaveMixed {
     }
 
     template <class Out>
-    __device__ static void run(const int s, const double* __restrict__ x,
+    __device__ __forceinline__ static void run(const int s, const double* __restrict__ x,
                                const double* __restrict__ y, const double* __restrict__ z,
                                const int* __restrict__ ja, Out out, double* lds,
                                int* __restrict__ info, uint8_t* __restrict__ status,
@@ -115,7 +89,7 @@ struct WaveMixed {
         int perm[NT];
         int mystep;
         float myrcp;
-        const int bad = lu32(a, perm, mystep, myrcp);
+        const int bad = lu32(a, perm, mystep, myrcp, reinterpret_cast<float*>(lds));
 
         // initial solve from the eliminated right-hand sides
         float r32[R];
@@ -144,7 +118,7 @@ struct WaveMixed {
 #pragma unroll
                     for (int c = 0; c < R; ++c) xl[c * 64 + mystep] = xd[c];
                 }
-                __syncthreads();
+                wave_lds_sync();
                 double rr[R];
 #pragma unroll
                 for (int c = 0; c < R; ++c) rr[c] = row[NT + c];
@@ -153,7 +127,7 @@ struct WaveMixed {
 #pragma unroll
                     for (int c = 0; c < R; ++c) rr[c] = fma(-row[j], xl[c * 64 + j], rr[c]);
                 }
-                __syncthreads();
+                wave_lds_sync();
 #pragma unroll
                 for (int c = 0; c < R; ++c) r32[c] = static_cast<float>(rr[c]);
                 solve32(a, perm, mystep, myrcp, r32);
@@ -162,8 +136,10 @@ struct WaveMixed {
 #pragma unroll
                 for (int c = 0; c < R; ++c) {
                     xd[c] = xd[c] + static_cast<double>(r32[c]);
-                    const float dm = wave_max(is_weight ? fabsf(r32[c]) : 0.f);
-                    const float xm = wave_max(is_weight ? static_cast<float>(fabs(xd[c])) : 0.f);
+                    const float dm = __uint_as_float(
+                        wave_umax(__float_as_uint(is_weight ? fabsf(r32[c]) : 0.f)));
+                    const float xm = __uint_as_float(wave_umax(
+                        __float_as_uint(is_weight ? static_cast<float>(fabs(xd[c])) : 0.f)));
                     const float ratio = (it > 0 && dprev[c] > 0.f) ? fminf(1.f, dm / dprev[c]) : 1.f;
                     if (!(dm * ratio <= static_cast<float>(kTol) * xm)) ok = false;
                     dprev[c] = dm;
@@ -180,7 +156,7 @@ struct WaveMixed {
         if (fallback) {  // FP64 solve of the kept FP64 row
             double xo[R];
             int ms;
-            inf = Base::lu_solve(row, ms, xo);
+            inf = Base::lu_solve(row, ms, xo, lds);
             mystep = ms;
 #pragma unroll
             for (int c = 0; c < R; ++c) xd[c] = xo[c];

@@ -228,6 +228,219 @@ __device__ __forceinline__ T wave_max(T v) {
     return v;
 }
 
+// Unsigned max over the wave: DPP within each 16-lane row (quad swaps, half
+// mirror, mirror), then the four row maxima through SGPRs (v_readlane +
+// s_max_u32). The result is wave-uniform.
+__device__ __forceinline__ unsigned wave_umax(unsigned v) {
+    v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0xB1, 0xF, 0xF, false)));
+    v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x4E, 0xF, 0xF, false)));
+    v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x141, 0xF, 0xF, false)));
+    v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x140, 0xF, 0xF, false)));
+    const unsigned a = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(v), 0));
+    const unsigned b = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(v), 16));
+    const unsigned c = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(v), 32));
+    const unsigned d = static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(v), 48));
+    return max(max(a, b), max(c, d));
+}
+
+// Lowest lane, among the candidates, of largest |a| (NaN never wins):
+// the same choice as a serial "first strict maximum" scan in lane order.
+// Returns -1 when there is no candidate. PV selects the reduction:
+// 0 = shuffle max of |a| then ballot; 1 = integer max on the bit pattern of
+// |a| (high word, then low word only when the high words tie).
+template <int PV>
+__device__ __forceinline__ int pivot_lane(double a, bool cand) {
+    cand = cand && !__builtin_isnan(a);
+    if constexpr (PV == 0) {
+        const double c = cand ? fabs(a) : -1.0;
+        const double mx = wave_max(c);
+        const unsigned long long hit = __ballot(cand && c == mx);
+        return hit ? __ffsll(hit) - 1 : -1;
+    } else {
+        const double fa = fabs(a);
+        const unsigned hi = cand ? static_cast<unsigned>(__double2hiint(fa)) : 0u;
+        const unsigned mh = wave_umax(hi);
+        unsigned long long hit = __ballot(cand && hi == mh);
+        if (__popcll(hit) > 1) {
+            const bool top = cand && hi == mh;
+            const unsigned lo = top ? static_cast<unsigned>(__double2loint(fa)) : 0u;
+            const unsigned ml = wave_umax(lo);
+            hit = __ballot(top && lo == ml);
+        }
+        return hit ? __ffsll(hit) - 1 : -1;
+    }
+}
+template <int PV>
+__device__ __forceinline__ int pivot_lane(float a, bool cand) {
+    cand = cand && !__builtin_isnan(a);
+    if constexpr (PV == 0) {
+        const float c = cand ? fabsf(a) : -1.f;
+        const float mx = wave_max(c);
+        const unsigned long long hit = __ballot(cand && c == mx);
+        return hit ? __ffsll(hit) - 1 : -1;
+    } else {
+        const unsigned key = cand ? __float_as_uint(fabsf(a)) : 0u;
+        const unsigned mk = wave_umax(key);
+        const unsigned long long hit = __ballot(cand && key == mk);
+        return hit ? __ffsll(hit) - 1 : -1;
+    }
+}
+
+// Orders this wave's LDS writes before its later LDS reads (LDS operations
+// of one wave complete in order; this keeps the compiler from moving them).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Pivot-row columns routed through LDS (the rest go lane-to-SGPR):
+// MODE 0 none, 1 all, 2 every other group of VW columns (VW = 16 bytes).
+template <int MODE, int VW>
+__device__ constexpr bool via_lds(int j) {
+    return MODE == 1 || (MODE == 2 && ((j / VW) & 1));
+}
+
+// Rank-1 update of columns J0..NC-1 of every lane's row with the pivot
+// row held by lane p: row[j] = fma(-l, u_j, row[j]). Columns go in chunks
+// of 8 (compile-time recursion, so every register index is a constant),
+// with a scheduling barrier between chunks so only one chunk of fetched
+// pivot-row values is live next to the row.
+template <int MODE, int NC, int J0, int G, typename T>
+__device__ __forceinline__ void publish_from(T* pb, const T (&row)[NC]) {
+    constexpr int VW = 16 / static_cast<int>(sizeof(T));
+    constexpr int j = G * VW;
+    if constexpr (j < NC) {
+        if constexpr (j + VW > J0 && via_lds<MODE, VW>(j)) {
+            if constexpr (j + VW <= NC) {
+                if constexpr (VW == 2) {
+                    *reinterpret_cast<double2*>(pb + j) = make_double2(row[j], row[j + 1]);
+                } else {
+                    *reinterpret_cast<float4*>(pb + j) =
+                        make_float4(row[j], row[j + 1], row[j + 2], row[j + 3]);
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < VW; ++q)
+                    if (j + q < NC) pb[j + q] = row[j + q];
+            }
+        }
+        publish_from<MODE, NC, J0, G + 1>(pb, row);
+    }
+}
+
+template <int MODE, int NC, int J0, int C0, typename T>
+__device__ __forceinline__ void update_from(T (&row)[NC], T l, int p, const T* pb) {
+    constexpr int VW = 16 / static_cast<int>(sizeof(T));
+    constexpr int CH = 8;
+    if constexpr (C0 < NC) {
+        T u[CH];
+#pragma unroll
+        for (int i = 0; i < CH; i += VW) {
+            const int j = C0 + i;
+            const bool vec = MODE != 0 && j + VW <= NC && j + VW > J0 && via_lds<MODE, VW>(j);
+            if (vec) {
+                if constexpr (VW == 2) {
+                    const double2 v = *reinterpret_cast<const double2*>(pb + j);
+                    u[i] = v.x;
+                    u[i + 1] = v.y;
+                } else {
+                    const float4 v = *reinterpret_cast<const float4*>(pb + j);
+                    u[i] = v.x;
+                    u[i + 1] = v.y;
+                    u[i + 2] = v.z;
+                    u[i + 3] = v.w;
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < VW; ++q) {
+                    if (j + q >= J0 && j + q < NC) {
+                        if (MODE != 0 && via_lds<MODE, VW>(j + q))
+                            u[i + q] = pb[j + q];
+                        else
+                            u[i + q] = bcast(row[j + q], p);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < CH; ++i)
+            if (C0 + i >= J0 && C0 + i < NC) row[C0 + i] = fma(-l, u[i], row[C0 + i]);
+        __builtin_amdgcn_sched_barrier(0);
+        update_from<MODE, NC, J0, C0 + CH>(row, l, p, pb);
+    }
+}
+
+template <int J0, int MODE, int NC, typename T>
+__device__ __forceinline__ void update_cols(T (&row)[NC], T l, int p, T* pb, bool me) {
+    if constexpr (MODE != 0) {
+        if (me) publish_from<MODE, NC, J0, 0>(pb, row);
+        wave_lds_sync();
+    }
+    update_from<MODE, NC, J0, J0 / 8 * 8>(row, l, p, pb);
+    if constexpr (MODE != 0) wave_lds_sync();  // reads done before the next publish
+}
+
+// State of the row-per-lane elimination.
+template <int NT, typename T>
+struct ElimState {
+    bool done = false;  // this lane's row has been a pivot row
+    int mystep = 64;    // step at which it was (its unknown's index)
+    T myrcp = T(0);     // 1 / its pivot
+    int info = 0;       // first zero (or NaN-only) pivot, 1-based
+    int perm[NT];       // lane of the pivot row of each step
+};
+
+// Elimination step K (compile time) of an NT x NT system with NC - NT
+// right-hand sides appended to the rows. Rows never move: the pivot row's
+// lane retires. STORE_L keeps the multiplier in row[K] (for later solves
+// with L), otherwise row[K] of every row still active becomes 0.
+template <int K, int NT, int NC, class Tune, bool STORE_L, typename T>
+__device__ __forceinline__ void elim_from(T (&row)[NC], ElimState<NT, T>& st, T* pb, int lane) {
+    if constexpr (K < NT) {
+        const bool active = lane < NT && !st.done;
+        const T a = row[K];
+        int p = pivot_lane<Tune::PV>(a, active);
+        if (p < 0) {  // every candidate NaN
+            p = __ffsll(__ballot(active)) - 1;
+            if (!st.info) st.info = K + 1;
+        }
+        p = __builtin_amdgcn_readfirstlane(p);
+        const T piv = bcast(a, p);
+        if (piv == T(0) && !st.info) st.info = K + 1;
+        const T rcp = T(1) / piv;
+        const bool is_piv = lane == p;
+        const bool elim = active && !is_piv;
+        const T l = elim ? a * rcp : T(0);
+        if constexpr (STORE_L) {
+            row[K] = elim ? l : a;
+        } else {
+            row[K] = active ? T(0) : a;
+        }
+        if (is_piv) {
+            st.done = true;
+            st.mystep = K;
+            st.myrcp = rcp;
+        }
+        // Materialise this step's select now: left alone, the compiler sinks
+        // all NT selects to the end and keeps NT reciprocals live meanwhile.
+        asm volatile("" : "+v"(st.myrcp), "+v"(st.mystep));
+        st.perm[K] = p;
+        update_cols<K + 1, Tune::BC>(row, l, p, pb, is_piv);
+        elim_from<K + 1, NT, NC, Tune, STORE_L>(row, st, pb, lane);
+    }
+}
+
+// Knobs of the wave LU, chosen per stencil shape by the autotuner:
+//   BC  pivot-row broadcast: 0 = v_readlane into SGPRs, 1 = LDS, 2 = half
+//       of the columns each way (balances the VALU and LDS pipes)
+//   PV  pivot search: see pivot_lane
+template <int BC_, int PV_>
+struct LuTune {
+    static constexpr int BC = BC_, PV = PV_;
+};
+using DefaultTune = LuTune<0, 0>;
+
 // Output table: reference kernels read a device table of NRHS pointers,
 // the C ABI passes them by value.
 template <typename T>
@@ -245,7 +458,7 @@ struct ValueOut {
 // One stencil on one wave64: gather -> fill -> LU -> solve -> scatter (FP64)
 // ---------------------------------------------------------------------------
 
-template <int DIM, int N, int P, int Q, int R, bool INTERP>
+template <int DIM, int N, int P, int Q, int R, bool INTERP, class Tune = DefaultTune>
 struct WaveAssembly {
     static constexpr int M = npoly_dim(DIM, P);
     static constexpr int NT = N + M;
@@ -255,22 +468,27 @@ struct WaveAssembly {
     static_assert(INTERP || R == DIM + 1, "derivative set: Laplacian + one per axis");
     static_assert(Q >= 3 && Q % 2 == 1, "odd PHS exponent >= 3");
 
-    // LDS per wave, in doubles: scaled points [DIM][N], monomials [M][N],
-    // interpolation centres [DIM][R] and their monomials [R][M].
-    static constexpr int lds_doubles = DIM * N + M * N + (INTERP ? DIM * R + R * M : 0);
+    using tune = Tune;
+    // LDS per wave, in doubles: pivot-row buffer [NC] (first, 16-byte
+    // aligned), scaled points [DIM][N], monomials [M][N], interpolation
+    // centres [DIM][R] and their monomials [R][M]; rounded up to an even
+    // count so consecutive waves' regions stay 16-byte aligned.
+    static constexpr int lds_pivot = (NC + 1) / 2 * 2;
+    static constexpr int lds_doubles =
+        (lds_pivot + DIM * N + M * N + (INTERP ? DIM * R + R * M : 0) + 1) / 2 * 2;
 
     // Gather, shift, scale and build this lane's row of [M | B]:
     // lane k < N holds point row k, lane N + t polynomial row t.
     template <typename T>
-    __device__ static void build_row(const int s, const double* __restrict__ x,
+    __device__ __forceinline__ static void build_row(const int s, const double* __restrict__ x,
                                      const double* __restrict__ y, const double* __restrict__ z,
                                      const int* __restrict__ ja_row, const double* __restrict__ cx,
                                      const double* __restrict__ cy, const double* __restrict__ cz,
                                      double* lds, T (&row)[NC], double& rho2_out) {
 #pragma clang fp contract(off)
         const int lane = lane_id();
-        double* pts = lds;               // [DIM][N]
-        double* polyT = lds + DIM * N;   // [M][N]
+        double* pts = lds + lds_pivot;   // [DIM][N]
+        double* polyT = pts + DIM * N;   // [M][N]
         double* cen = polyT + M * N;     // [DIM][R]
         double* cmono = cen + DIM * R;   // [R][M]
 
@@ -309,20 +527,32 @@ struct WaveAssembly {
                 for (int t = 0; t < M; ++t) cmono[lane * M + t] = cm[t];
             }
         }
-        __syncthreads();
+        wave_lds_sync();
 
         constexpr PHS<Q> phi{};
+        // Stencil point col comes from lane col through scalar registers
+        // (v_readlane), not LDS: it stays out of the vector registers.
+        double bx[N], by[N], bz[DIM == 3 ? N : 1];
+#pragma unroll
+        for (int col = 0; col < N; ++col) {
+            bx[col] = bcast(xs, col);
+            by[col] = bcast(ys, col);
+            if constexpr (DIM == 3) bz[col] = bcast(zs, col);
+        }
         if (lane < N) {
 #pragma unroll
             for (int col = 0; col < N; ++col) {
-                const double dx = pts[col] - xs;
-                const double dy = pts[N + col] - ys;
+                const double dx = bx[col] - xs;
+                const double dy = by[col] - ys;
                 double r2 = dx * dx + dy * dy;
                 if constexpr (DIM == 3) {
-                    const double dz = pts[2 * N + col] - zs;
+                    const double dz = bz[col] - zs;
                     r2 = r2 + dz * dz;
                 }
                 row[col] = static_cast<T>(phi(r2));
+                // keep the scheduler from interleaving all N evaluations
+                // (it would hold N sqrt temporaries live at once)
+                if ((col & 3) == 3) __builtin_amdgcn_sched_barrier(0);
             }
 #pragma unroll
             for (int t = 0; t < M; ++t) row[N + t] = static_cast<T>(mono[t]);
@@ -373,63 +603,31 @@ struct WaveAssembly {
     // Right-looking partial-pivot elimination of [M | B] held one row per
     // lane, then back substitution. On return lane i holds the solution
     // of unknown mystep in xout[c]. Returns info (0, or first zero pivot).
-    __device__ static int lu_solve(double (&row)[NC], int& mystep, double (&xout)[R]) {
+    __device__ __forceinline__ static int lu_solve(double (&row)[NC], int& mystep, double (&xout)[R],
+                                   double* lds) {
 #pragma clang fp contract(off)
         const int lane = lane_id();
-        bool done = false;
-        double myrcp = 0.0;
-        int info = 0;
-        int perm[NT];
-        mystep = 64;
-#pragma unroll
-        for (int k = 0; k < NT; ++k) {
-            const bool active = lane < NT && !done;
-            const double a = row[k];
-            const double cand = active ? fabs(a) : -1.0;
-            const double mx = wave_max(cand);
-            const unsigned long long hit = __ballot(active && cand == mx);
-            int p;
-            if (hit) {
-                p = __ffsll(hit) - 1;
-            } else {  // every candidate NaN
-                p = __ffsll(__ballot(active)) - 1;
-                if (!info) info = k + 1;
-            }
-            p = __builtin_amdgcn_readfirstlane(p);
-            const double piv = bcast(a, p);
-            if (piv == 0.0 && !info) info = k + 1;
-            const double rcp = 1.0 / piv;
-            const bool is_piv = lane == p;
-            const double l = (active && !is_piv) ? a * rcp : 0.0;
-            row[k] = active ? 0.0 : row[k];
-            if (is_piv) {
-                done = true;
-                mystep = k;
-                myrcp = rcp;
-            }
-            perm[k] = p;
-#pragma unroll
-            for (int j = k + 1; j < NC; ++j) {
-                const double u = bcast(row[j], p);
-                row[j] = fma(-l, u, row[j]);
-            }
-        }
+        ElimState<NT, double> st;
+        elim_from<0, NT, NC, Tune, false>(row, st, lds, lane);
+        // back substitution: row[k] holds u_ik for rows pivotal before step
+        // k and 0 for every other row, so all lanes update alike
 #pragma unroll
         for (int k = NT - 1; k >= 0; --k) {
-            const int p = perm[k];
+            const int p = st.perm[k];
 #pragma unroll
             for (int c = 0; c < R; ++c) {
-                const double t = bcast(row[NT + c] * myrcp, p);
+                const double t = bcast(row[NT + c] * st.myrcp, p);
                 row[NT + c] = fma(-row[k], t, row[NT + c]);
             }
         }
 #pragma unroll
-        for (int c = 0; c < R; ++c) xout[c] = row[NT + c] * myrcp;
-        return info;
+        for (int c = 0; c < R; ++c) xout[c] = row[NT + c] * st.myrcp;
+        mystep = st.mystep;
+        return st.info;
     }
 
     template <class Out>
-    __device__ static void run(const int s, const double* __restrict__ x,
+    __device__ __forceinline__ static void run(const int s, const double* __restrict__ x,
                                const double* __restrict__ y, const double* __restrict__ z,
                                const int* __restrict__ ja, Out out,
                                const double* __restrict__ cx, const double* __restrict__ cy,
@@ -442,7 +640,7 @@ struct WaveAssembly {
         build_row<double>(s, x, y, z, ja_row, cx, cy, cz, lds, row, rho2);
         int mystep;
         double xo[R];
-        const int inf = lu_solve(row, mystep, xo);
+        const int inf = lu_solve(row, mystep, xo, lds);
         if (mystep < N) {
             const double rho = sqrt(rho2);
             const size_t base = static_cast<size_t>(s) * N + mystep;

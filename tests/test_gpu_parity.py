@@ -69,6 +69,42 @@ def test_diff_ops_match_oracle(m, n, P, Q, prec):
         assert ((status & 0x3) <= 3).all()
 
 
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_every_variant_matches_oracle(prec):
+    n, P, Q = 35, 3, 5
+    plan = R.Plan(2, n, P, Q, "diff", precision=prec)
+    x, y = R.jittered_grid(2, 48, 0.25, seed=11)
+    ja = R.knn_stencils(n, x, y)
+    wref, _ = O.assemble(2, n, P, Q, x, y, ja)
+    tol = FP64_TOL if prec == "fp64" else MIXED_TOL
+    first = None
+    for v in range(plan.num_variants):
+        plan.variant = v
+        w, info, _ = _run(plan, x, y, ja)
+        assert (info == 0).all()
+        e = _relerr(w, wref, n)
+        print(f"{prec} variant {v} ({plan.variant_name()}): max rel err {e.max():.3e}")
+        assert e.max() <= tol
+        if prec == "fp64":      # variants differ in speed, not in results
+            if first is None:
+                first = w
+            else:
+                for a, b in zip(w, first):
+                    np.testing.assert_array_equal(a, b)
+
+
+def test_autotune_picks_a_variant():
+    import torch
+    n, P, Q = 21, 2, 3
+    plan = R.Plan(2, n, P, Q, "diff")
+    x, y = R.jittered_grid(2, 64, 0.25, seed=12)
+    ja = R.knn_stencils(n, x, y)
+    w = [torch.empty(ja.size, dtype=torch.float64, device="cuda:0") for _ in range(3)]
+    ms = plan.autotune(x.size, _dev(x), _dev(y), _dev(ja), w, reps=2)
+    assert len(ms) == plan.num_variants and all(t > 0 for t in ms)
+    assert plan.variant == int(np.argmin(ms))
+
+
 def test_interp_d2q9_matches_oracle():
     """Interpolation to the 9 D2Q9 departure points of each node, the
     reference demo's NRHS = 9 (main.f90:16,55: centres 0.5*(x, y))."""
