@@ -15,35 +15,12 @@
 #include "rbf_assembly.h"
 #include "rbf_operators.h"
 #include "rbf_mixed.h"
+#include "assemble_common.h"
 
 namespace {
 
 using namespace rbf_operators;
-
-constexpr int kMaxOps = 16;
-
-struct LaunchArgs {
-    int32_t nstencils;
-    const double *x, *y, *z;
-    const int32_t* ja;
-    double* w[kMaxOps];
-    const double *cx, *cy, *cz;
-    int32_t* info;
-    uint8_t* status;
-    int32_t max_refine;
-    hipStream_t stream;
-};
-
-// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs,
-// each with its own L2. Renumber so each XCD works on one contiguous range
-// of (space-filling-curve ordered) stencils and its L2 sees neighbouring
-// coordinates: block b runs block-range (b % 8) * chunk + b / 8.
-__device__ __forceinline__ int xcd_block(int nblocks) {
-    const int b = static_cast<int>(blockIdx.x);
-    const int chunk = (nblocks + 7) / 8;
-    return (b & 7) * chunk + (b >> 3);
-}
-inline unsigned xcd_grid(int nblocks) { return 8u * static_cast<unsigned>((nblocks + 7) / 8); }
+using namespace rbf_amd_internal;
 
 // WPB waves per workgroup, one stencil per wave, each wave with its own
 // LDS region.
@@ -138,6 +115,7 @@ struct Shape {
     int dim, n, P, Q, op_set, nrhs, precision;
     int nt, nops;
     const Variant* variants;
+    int nvariants;
 };
 
 template <int DIM, int N, int P, int Q>
@@ -160,19 +138,30 @@ template <int DIM, int N, int P, int Q>
 constexpr Shape diff64() {
     using E = detail::WaveAssembly<DIM, N, P, Q, DIM + 1, false>;
     return {DIM, N, P, Q, RBFA_OPS_DIFF, 0, RBFA_PREC_FP64, E::NT, DIM + 1,
-            VariantSet<Fp64L, DiffEng<DIM, N, P, Q>::template E>::list};
+            VariantSet<Fp64L, DiffEng<DIM, N, P, Q>::template E>::list, kVariants};
+}
+template <int DIM, int N, int P, int Q>
+struct GroupVariants {
+    static constexpr Variant list[1] = {
+        {"workgroup nw=2 bc=lds pv=dpp", &launch_group_diff<DIM, N, P, Q>}};
+};
+template <int DIM, int N, int P, int Q>
+constexpr Shape diffgroup() {
+    constexpr int M = npoly_dim(DIM, P);
+    return {DIM, N, P, Q, RBFA_OPS_DIFF, 0, RBFA_PREC_FP64, N + M, DIM + 1,
+            GroupVariants<DIM, N, P, Q>::list, 1};
 }
 template <int N, int P, int Q, int R>
 constexpr Shape interp64() {
     using E = detail::WaveAssembly<2, N, P, Q, R, true>;
     return {2, N, P, Q, RBFA_OPS_INTERP, R, RBFA_PREC_FP64, E::NT, R,
-            VariantSet<Fp64L, InterpEng<N, P, Q, R>::template E>::list};
+            VariantSet<Fp64L, InterpEng<N, P, Q, R>::template E>::list, kVariants};
 }
 template <int DIM, int N, int P, int Q>
 constexpr Shape diffmix() {
     using E = detail::WaveMixed<DIM, N, P, Q>;
     return {DIM, N, P, Q, RBFA_OPS_DIFF, 0, RBFA_PREC_MIXED, E::NT, DIM + 1,
-            VariantSet<MixL, MixEng<DIM, N, P, Q>::template E>::list};
+            VariantSet<MixL, MixEng<DIM, N, P, Q>::template E>::list, kVariants};
 }
 
 // The shapes this library ships: the stencil families of the workloads it
@@ -184,6 +173,8 @@ const Shape kShapes[] = {
     diff64<2, 35, 3, 3>(),  diff64<2, 21, 2, 5>(), diff64<2, 13, 1, 3>(),
     interp64<9, 2, 3, 9>(), interp64<21, 2, 3, 9>(),
     diffmix<2, 21, 2, 3>(), diffmix<2, 35, 3, 5>(), diffmix<2, 13, 2, 3>(),
+    // systems larger than a wavefront: one two-wave workgroup per stencil
+    diffgroup<2, 56, 4, 7>(), diffgroup<3, 80, 3, 3>(),
 };
 
 }  // namespace
@@ -266,17 +257,17 @@ RBFA_API int32_t rbfa_assemble(const rbfa_plan* plan, int32_t nstencils, const d
     return sh.variants[plan->variant].fn(a) == hipSuccess ? RBFA_OK : RBFA_ERR_HIP;
 }
 
-RBFA_API int32_t rbfa_plan_num_variants(const rbfa_plan* plan) { return plan ? kVariants : -1; }
+RBFA_API int32_t rbfa_plan_num_variants(const rbfa_plan* plan) { return plan ? plan->shape.nvariants : -1; }
 
 RBFA_API const char* rbfa_plan_variant_name(const rbfa_plan* plan, int32_t v) {
-    if (!plan || v < 0 || v >= kVariants) return nullptr;
+    if (!plan || v < 0 || v >= plan->shape.nvariants) return nullptr;
     return plan->shape.variants[v].name;
 }
 
 RBFA_API int32_t rbfa_plan_get_variant(const rbfa_plan* plan) { return plan ? plan->variant : -1; }
 
 RBFA_API int32_t rbfa_plan_set_variant(rbfa_plan* plan, int32_t v) {
-    if (!plan || v < 0 || v >= kVariants) return RBFA_ERR_ARG;
+    if (!plan || v < 0 || v >= plan->shape.nvariants) return RBFA_ERR_ARG;
     plan->variant = v;
     return RBFA_OK;
 }
@@ -293,7 +284,7 @@ RBFA_API int32_t rbfa_plan_autotune(rbfa_plan* plan, int32_t nstencils, const do
     int best = plan->variant;
     float best_ms = 0.f;
     int32_t rc = RBFA_OK;
-    for (int v = 0; v < kVariants && rc == RBFA_OK; ++v) {
+    for (int v = 0; v < plan->shape.nvariants && rc == RBFA_OK; ++v) {
         plan->variant = v;
         rc = rbfa_assemble(plan, nstencils, x, y, z, ja, w, xc, yc, zc, info, status, stream);
         if (rc != RBFA_OK) break;

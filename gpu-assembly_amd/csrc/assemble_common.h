@@ -1,0 +1,44 @@
+// assemble_common.h -- internal to librbf_amd: launch arguments shared by
+// the kernel translation units, and the XCD-aware block numbering.
+#ifndef RBF_ASSEMBLE_COMMON_H
+#define RBF_ASSEMBLE_COMMON_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace rbf_amd_internal {
+
+constexpr int kMaxOps = 16;
+
+struct LaunchArgs {
+    int32_t nstencils;
+    const double *x, *y, *z;
+    const int32_t* ja;
+    double* w[kMaxOps];
+    const double *cx, *cy, *cz;
+    int32_t* info;
+    uint8_t* status;
+    int32_t max_refine;
+    hipStream_t stream;
+};
+
+// MI355X dispatches consecutive workgroups round-robin over its 8 XCDs,
+// each with its own L2. Renumber so each XCD works on one contiguous range
+// of (space-filling-curve ordered) stencils and its L2 sees neighbouring
+// coordinates: block b runs block-range (b % 8) * chunk + b / 8.
+__device__ __forceinline__ int xcd_block(int nblocks) {
+    const int b = static_cast<int>(blockIdx.x);
+    const int chunk = (nblocks + 7) / 8;
+    return (b & 7) * chunk + (b >> 3);
+}
+inline unsigned xcd_grid(int nblocks) { return 8u * static_cast<unsigned>((nblocks + 7) / 8); }
+
+// Systems larger than 64: one two-wave workgroup per stencil (defined in
+// assemble_group.hip, instantiated there for the shipped shapes).
+template <int DIM, int N, int P, int Q>
+hipError_t launch_group_diff(const LaunchArgs& a);
+
+}  // namespace rbf_amd_internal
+
+#endif  // RBF_ASSEMBLE_COMMON_H

@@ -1,0 +1,40 @@
+// assemble_group.hip -- librbf_amd kernels for stencil systems larger than
+// a wavefront (one two-wave workgroup per stencil, include/rbf_group.h),
+// in a translation unit of their own: they are the slowest to compile.
+#include <hip/hip_runtime.h>
+
+#include "assemble_common.h"
+#include "rbf_group.h"
+
+namespace rbf_amd_internal {
+
+using namespace rbf_operators;
+
+template <class E, int NW>
+__global__ void __launch_bounds__(64 * NW)
+k_assemble_group(int nstencils, const double* __restrict__ x, const double* __restrict__ y,
+                 const double* __restrict__ z, const int* __restrict__ ja,
+                 detail::ValueOut<double, E::nrhs> out, int* __restrict__ info,
+                 uint8_t* __restrict__ status) {
+    __shared__ __align__(16) double lds[E::lds_doubles];
+    const int s = xcd_block(nstencils);
+    if (s >= nstencils) return;
+    E::run(s, x, y, z, ja, out, lds, info);
+    if (status && threadIdx.x == 0) status[s] = 0;
+}
+
+template <int DIM, int N, int P, int Q>
+hipError_t launch_group_diff(const LaunchArgs& a) {
+    constexpr int NW = 2;
+    using E = detail::GroupAssembly<DIM, N, P, Q, DIM + 1, false, NW>;
+    detail::ValueOut<double, E::nrhs> out;
+    for (int c = 0; c < E::nrhs; ++c) out.p[c] = a.w[c];
+    hipLaunchKernelGGL((k_assemble_group<E, NW>), dim3(xcd_grid(a.nstencils)), dim3(64 * NW), 0,
+                       a.stream, a.nstencils, a.x, a.y, a.z, a.ja, out, a.info, a.status);
+    return hipGetLastError();
+}
+
+template hipError_t launch_group_diff<2, 56, 4, 7>(const LaunchArgs&);
+template hipError_t launch_group_diff<3, 80, 3, 3>(const LaunchArgs&);
+
+}  // namespace rbf_amd_internal

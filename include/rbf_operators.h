@@ -385,7 +385,7 @@ __device__ __forceinline__ void update_cols(T (&row)[NC], T l, int p, T* pb, boo
 template <int NT, typename T>
 struct ElimState {
     bool done = false;  // this lane's row has been a pivot row
-    int mystep = 64;    // step at which it was (its unknown's index)
+    int mystep = 1 << 20;  // step at which it was (its unknown's index); never, for padding lanes
     T myrcp = T(0);     // 1 / its pivot
     int info = 0;       // first zero (or NaN-only) pivot, 1-based
     int perm[NT];       // lane of the pivot row of each step

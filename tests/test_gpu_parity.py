@@ -144,3 +144,19 @@ def test_singular_stencil_reports_info():
     ok = iref == 0
     assert _relerr([a.reshape(-1, n)[ok] for a in w], [b.reshape(-1, n)[ok] for b in wref],
                    n).max() <= FP64_TOL
+
+
+@pytest.mark.parametrize("dim,m,n,P,Q", [(2, 32, 56, 4, 7), (3, 10, 80, 3, 3)])
+def test_large_systems_match_oracle(dim, m, n, P, Q):
+    """Systems larger than a wavefront (71x71 in 2-D, 100x100 in 3-D), one
+    two-wave workgroup per stencil."""
+    plan = R.Plan(dim, n, P, Q, "diff")
+    pts = R.jittered_grid(dim, m, 0.25, seed=13)
+    ja = R.knn_stencils(n, *pts)
+    z = pts[2] if dim == 3 else None
+    w, info, status = _run(plan, pts[0], pts[1], ja, z=z)
+    wref, iref = O.assemble(dim, n, P, Q, pts[0], pts[1], ja, z=z)
+    assert (info == 0).all() and (iref == 0).all()
+    e = _relerr(w, wref, n)
+    print(f"dim={dim} n={n} nt={plan.nt}: max rel err {e.max():.3e}")
+    assert e.max() <= FP64_TOL
