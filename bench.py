@@ -8,9 +8,11 @@ d/dy into CSR (FP64 weights, int32 column indices shared with ja).
 One step = one assembly pass over all 1M stencils, inputs resident in HBM.
 Both forms are timed: the plain FP64 kernel and the mixed form (FP32 LU +
 FP64 refinement, <= 3 steps, FP64 fallback). The headline value is the
-mixed form's stencils/s; the FP64 form, their ratio, the weight errors of
-both against the CPU FP64 oracle, the fallback fraction and the analytic
-consistency error are reported beside it.
+stencils/s of the faster form whose weights stay within 1e-8 (max relative,
+per stencil and operator) of the CPU FP64 oracle over the whole workload;
+both forms, their ratio, the weight errors, the fallback fraction and the
+analytic consistency error are reported beside it. The oracle run doubles
+as the CPU baseline (all host cores, same 1M stencils).
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3]
 """
@@ -80,12 +82,16 @@ def hip_timed(fn, steps, warmup):
 
 
 def read_traffic(kernel_tag):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if any."""
-    path = os.path.join(ROOT, "profiles", "traffic_" + kernel_tag + ".json")
+    """HBM bytes per launch of the kernel, from the committed rocprofv3 --pmc
+    summary (profiles/<kernel_tag>.md, line "traffic_bytes_per_launch: N")."""
+    path = os.path.join(ROOT, "profiles", kernel_tag + ".md")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("bytes_per_launch")
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    return None
 
 
 def main():
@@ -94,11 +100,15 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--cpu-sample", type=int, default=65536,
-                    help="stencils in the CPU-baseline / oracle-parity sample")
+    ap.add_argument("--cpu-sample", type=int, default=1 << 20,
+                    help="stencils in the CPU-baseline / oracle-parity sample (default: all)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity")
     ap.add_argument("--form", choices=["both", "fp64", "mixed"], default="both")
     ap.add_argument("--no-tune", action="store_true", help="skip the variant autotuner")
+    ap.add_argument("--variant", type=int, default=None,
+                    help="use this kernel variant (no autotuning)")
+    ap.add_argument("--profile-copy", action="store_true",
+                    help="also run 3 plain 256 MiB device copies (HBM counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -133,7 +143,9 @@ def main():
             plan.assemble(nst, x, y, ja, w, info=info, status=status)
 
         tune_ms = None
-        if not args.no_tune:
+        if args.variant is not None:
+            plan.variant = args.variant
+        elif not args.no_tune:
             tune_ms = plan.autotune(nst, x, y, ja, w, info=info, status=status, reps=2)
             log(f"[{form}] autotune: " + ", ".join(
                 f"{plan.variant_name(v)}={t:.3f}ms" for v, t in enumerate(tune_ms)))
@@ -157,7 +169,9 @@ def main():
         del w
         torch.cuda.empty_cache()
 
-    head = "mixed" if "mixed" in res else "fp64"
+    # headline: the faster form whose weights meet the 1e-8 bar (checked
+    # against the oracle below; without the oracle run, the faster form)
+    head = min(res, key=lambda f: res[f]["ms"])
     nt, r = res[head]["nt"], res[head]["r"]
     flop = workloads.flops_per_stencil(nt, r)
     byt = workloads.bytes_per_stencil(n, r)
@@ -183,8 +197,8 @@ def main():
     if not args.no_cpu:
         from oracle import oracle as O
         cores = len(os.sched_getaffinity(0))
-        s0 = nst // 2
-        cnt = min(args.cpu_sample, nst - s0)
+        cnt = min(args.cpu_sample, nst)
+        s0 = (nst - cnt) // 2
         log(f"CPU oracle: {cnt} stencils on {cores} threads")
         t0 = time.perf_counter()
         wref, iref = O.assemble(2, n, P, Q, W["x"], W["y"], W["ja"], s0=s0, count=cnt,
@@ -203,7 +217,22 @@ def main():
             extra[form]["consistency_rel_err"] = consistency(d["w"], W["ja"], W["x"], W["y"], n)
             log(f"[{form}] weight rel err max {e.max():.3e} median {np.median(e):.3e}")
 
-    kernel_tag = "assemble_%s_n35" % head
+    ok = [f for f in res if extra[f].get("weight_rel_err_max", 0.0) <= 1e-8]
+    if ok and head not in ok:
+        head = min(ok, key=lambda f: res[f]["ms"])
+        nt, r = res[head]["nt"], res[head]["r"]
+        ms = res[head]["ms"]
+        value = nst / (ms * 1e-3)
+        achieved_tflops = flop * nst / (ms * 1e-3) / 1e12
+    if args.profile_copy:
+        # a plain device copy of known size, for scaling the HBM counters
+        src = torch.empty(32 << 20, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 3 x 256 MiB device copies")
+    kernel_tag = "assemble_%s_n35_1m" % head
     line = {
         "metric": "stencils/s assembled, 1M nodes n=35; mixed vs FP64 at <=1e-8 weight rel. err",
         "value": value,
@@ -216,6 +245,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32+fp64" if head == "mixed" else "f64",
+        "form": head,
         "data": "synthetic (seeded jittered grid, Hilbert order, host kNN)",
         "config": {"workload": "square_1m_n35: 1024x1024 jittered unit square, Hilbert, "
                                "n=35, PHS r^5 + P3 (45x45), Laplacian/dx/dy, form=" + head,
