@@ -37,7 +37,8 @@ k_assemble_fp64(int nstencils, const double* __restrict__ x, const double* __res
     const int w = static_cast<int>(threadIdx.x >> 6);
     const int s = b * WPB + w;
     if (b >= nblocks || s >= nstencils) return;
-    E::run(s, x, y, z, ja, out, cx, cy, cz, lds + w * E::lds_doubles, info);
+    E::run(s, static_cast<size_t>(s) * E::nsten, x, y, z, ja, out, cx, cy, cz,
+           lds + w * E::lds_doubles, info);
     if (status && detail::lane_id() == 0) status[s] = 0;
 }
 
@@ -65,7 +66,8 @@ k_assemble_mixed(int nstencils, const double* __restrict__ x, const double* __re
     const int w = static_cast<int>(threadIdx.x >> 6);
     const int s = b * WPB + w;
     if (b >= nblocks || s >= nstencils) return;
-    E::run(s, x, y, z, ja, out, lds + w * E::lds_doubles, info, status, max_refine);
+    E::run(s, static_cast<size_t>(s) * E::nsten, x, y, z, ja, out, lds + w * E::lds_doubles, info,
+           status, max_refine);
 }
 
 template <class E, int WPB>

@@ -19,7 +19,7 @@ k_assemble_group(int nstencils, const double* __restrict__ x, const double* __re
     __shared__ __align__(16) double lds[E::lds_doubles];
     const int s = xcd_block(nstencils);
     if (s >= nstencils) return;
-    E::run(s, x, y, z, ja, out, lds, info);
+    E::run(s, static_cast<size_t>(s) * E::nsten, x, y, z, ja, out, lds, info);
     if (status && threadIdx.x == 0) status[s] = 0;
 }
 

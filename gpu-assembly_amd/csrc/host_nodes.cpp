@@ -339,6 +339,41 @@ int32_t permute_impl(int32_t n, const int32_t* perm, T* a) {
     return RBFA_OK;
 }
 
+
+// Graded unit-disk node set: concentric rings with spacing h(r) = hc (1 -
+// 0.75 r) both along and across the rings (centre spacing hc, rim hc/4),
+// a random phase per ring and a jitter of +-0.15 h in radius and angle, so
+// neighbouring nodes keep a minimum distance (a Poisson-disk-like set).
+// Returns the number of interior nodes a centre spacing hc gives.
+static int64_t disk_rings(double hc, uint64_t seed, double* x, double* y, int64_t cap) {
+    XorShift64 rng(seed);
+    int64_t cnt = 0;
+    auto emit = [&](double px, double py) {
+        if (x && cnt < cap) {
+            x[cnt] = px;
+            y[cnt] = py;
+        }
+        ++cnt;
+    };
+    emit(0.0, 0.0);
+    const double two_pi = 6.283185307179586;
+    double r = 0.0;
+    while (true) {
+        r += hc * (1.0 - 0.75 * r);
+        const double h = hc * (1.0 - 0.75 * r);
+        if (r > 1.0 - 0.5 * h) break;
+        const int64_t m = std::max<int64_t>(6, static_cast<int64_t>(std::llround(two_pi * r / h)));
+        const double phase = rng.next();
+        for (int64_t j = 0; j < m; ++j) {
+            const double dr = 0.15 * h * (2.0 * rng.next() - 1.0);
+            const double dt = 0.15 * (2.0 * rng.next() - 1.0);
+            const double th = two_pi * (static_cast<double>(j) + phase + dt) / static_cast<double>(m);
+            emit((r + dr) * std::cos(th), (r + dr) * std::sin(th));
+        }
+    }
+    return cnt;
+}
+
 }  // namespace
 
 extern "C" {
@@ -471,6 +506,48 @@ RBFA_API int32_t rbfa_gen_jittered_grid(int32_t dim, int32_t m, double jitter, u
                 y[p] = (j + 0.5) * h + jitter * h * (2.0 * rng.next() - 1.0);
                 if (dim == 3) z[p] = (kk + 0.5) * h + jitter * h * (2.0 * rng.next() - 1.0);
             }
+    return RBFA_OK;
+}
+
+
+RBFA_API int32_t rbfa_gen_graded_disk(int32_t n_interior, int32_t n_boundary, uint64_t seed,
+                                      double* x, double* y, int32_t* flag) {
+    if (n_interior < 1 || n_boundary < 0 || !x || !y || !flag) return RBFA_ERR_ARG;
+    // centre spacing for at least n_interior ring nodes (bisection on hc)
+    double lo = 1e-6, hi = 0.5;
+    if (disk_rings(hi, seed, nullptr, nullptr, 0) >= n_interior) lo = hi;
+    for (int it = 0; it < 80 && hi - lo > 1e-15 * hi; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (disk_rings(mid, seed, nullptr, nullptr, 0) >= n_interior) lo = mid; else hi = mid;
+    }
+    const int64_t have = disk_rings(lo, seed, nullptr, nullptr, 0);
+    std::vector<double> bx(static_cast<size_t>(have)), by(static_cast<size_t>(have));
+    disk_rings(lo, seed, bx.data(), by.data(), have);
+    // drop the surplus nodes at seeded random positions
+    std::vector<char> keep(static_cast<size_t>(have), 1);
+    XorShift64 rng(seed ^ 0xA5A5A5A5A5A5A5A5ull);
+    for (int64_t extra = have - n_interior; extra > 0;) {
+        const int64_t i = 1 + static_cast<int64_t>(rng.next() * static_cast<double>(have - 1));
+        if (keep[i]) {
+            keep[i] = 0;
+            --extra;
+        }
+    }
+    int64_t o = 0;
+    for (int64_t i = 0; i < have; ++i)
+        if (keep[i]) {
+            x[o] = bx[i];
+            y[o] = by[i];
+            flag[o] = 0;
+            ++o;
+        }
+    const double two_pi = 6.283185307179586;
+    for (int32_t j = 0; j < n_boundary; ++j, ++o) {
+        const double th = two_pi * j / n_boundary;
+        x[o] = std::cos(th);
+        y[o] = std::sin(th);
+        flag[o] = 1;
+    }
     return RBFA_OK;
 }
 

@@ -84,6 +84,16 @@ def lib():
         "rbfa_plan_get_variant": (i32, [P]),
         "rbfa_plan_set_variant": (i32, [P, i32]),
         "rbfa_plan_autotune": (i32, [P, i32, P, P, P, P, P, P, P, P, P, P, P, i32, P]),
+        "rbfa_gen_graded_disk": (i32, [i32, i32, u64, P, P, P]),
+        "rbfa_batch_create": (i32, [i32, P, P]),
+        "rbfa_batch_destroy": (i32, [P]),
+        "rbfa_batch_bind": (i32, [P, i32, P, P, P, P, P, P, P, P, P, P]),
+        "rbfa_batch_class_count": (i32, [P, i32]),
+        "rbfa_batch_class_variant": (ctypes.c_char_p, [P, i32]),
+        "rbfa_batch_set_mode": (i32, [P, i32]),
+        "rbfa_batch_get_mode": (i32, [P]),
+        "rbfa_batch_launch": (i32, [P, P]),
+        "rbfa_batch_autotune": (i32, [P, P, i32, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -262,6 +272,29 @@ def jittered_grid(dim, m, jitter=0.25, seed=1):
     return (x, y) if dim == 2 else (x, y, z)
 
 
+def graded_disk(n_interior, n_boundary, seed=1):
+    """Graded unit disk (ring spacing hc at the centre to hc/4 at the rim,
+    jittered) plus n_boundary equispaced rim nodes; returns x, y, flag."""
+    n = int(n_interior) + int(n_boundary)
+    x, y = np.empty(n), np.empty(n)
+    flag = np.empty(n, np.int32)
+    _check(lib().rbfa_gen_graded_disk(int(n_interior), int(n_boundary), int(seed), _ptr(x),
+                                      _ptr(y), _ptr(flag)), "rbfa_gen_graded_disk")
+    return x, y, flag
+
+
+def ragged_stencils(sizes, x, y, z=None):
+    """kNN stencils of per-node size sizes[s] as a ragged CSR graph (ia, ja):
+    the first sizes[s] of each node's nearest neighbours (self first)."""
+    sizes = np.ascontiguousarray(sizes, np.int32)
+    kmax = int(sizes.max())
+    full = knn_stencils(kmax, x, y, z).reshape(-1, kmax)
+    ia = np.zeros(sizes.size + 1, np.int32)
+    np.cumsum(sizes, out=ia[1:])
+    mask = np.arange(kmax)[None, :] < sizes[:, None]
+    return ia, np.ascontiguousarray(full[mask])
+
+
 class NodeSet:
     """Node set with a renumbering chain (rbf_nodeset.h:46-235)."""
 
@@ -304,6 +337,71 @@ class NodeSet:
 # ---------------------------------------------------------------------------
 # Device assembly
 # ---------------------------------------------------------------------------
+
+class Batch:
+    """An uneven batch over size classes [(n, P, Q), ...] (2-D, FP64,
+    Laplacian / d/dx / d/dy), bound to one ragged graph and captured into a
+    hipGraph; launch() replays it."""
+
+    def __init__(self, classes):
+        arr = (_PlanDesc * len(classes))(*[_PlanDesc(2, n, P, Q, OPS_DIFF, 0, PREC_FP64, 3)
+                                           for n, P, Q in classes])
+        self._h = ctypes.c_void_p()
+        _check(lib().rbfa_batch_create(len(classes), arr, ctypes.byref(self._h)),
+               "rbfa_batch_create")
+        self.classes = sorted(classes)
+        self._keep = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.rbfa_batch_destroy(h)
+            self._h = None
+
+    def bind(self, ia_host, x, y, ia, ja, w, info=None, status=None, stream=None):
+        import torch
+
+        def dp(t):
+            return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+        ia_host = _i32(ia_host)
+        tab = (ctypes.c_void_p * 3)(*[t.data_ptr() for t in w])
+        self._keep = (ia_host, tab, x, y, ia, ja, w, info, status)
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        _check(lib().rbfa_batch_bind(self._h, ia_host.size - 1, _ptr(ia_host), dp(x), dp(y), None,
+                                     dp(ia), dp(ja), tab, dp(info), dp(status),
+                                     ctypes.c_void_p(stream)), "rbfa_batch_bind")
+
+    def class_count(self, c):
+        return lib().rbfa_batch_class_count(self._h, int(c))
+
+    def class_variant(self, c):
+        return lib().rbfa_batch_class_variant(self._h, int(c)).decode()
+
+    @property
+    def mode(self):
+        return lib().rbfa_batch_get_mode(self._h)
+
+    @mode.setter
+    def mode(self, m):
+        _check(lib().rbfa_batch_set_mode(self._h, int(m)), "rbfa_batch_set_mode")
+
+    def launch(self, stream=None):
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        _check(lib().rbfa_batch_launch(self._h, ctypes.c_void_p(stream)), "rbfa_batch_launch")
+
+    def autotune(self, reps=3, stream=None):
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        ms = (ctypes.c_float * 2)()
+        _check(lib().rbfa_batch_autotune(self._h, ctypes.c_void_p(stream), int(reps), ms),
+               "rbfa_batch_autotune")
+        return list(ms)
+
 
 class Plan:
     """A compiled stencil shape: dim, stencil size n, degree P, PHS r^Q,

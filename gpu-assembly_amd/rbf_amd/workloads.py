@@ -9,7 +9,7 @@ import time
 
 import numpy as np
 
-from . import hilbert_order, jittered_grid, knn_stencils, morton_order_3d
+from . import graded_disk, hilbert_order, jittered_grid, knn_stencils, morton_order_3d, ragged_stencils
 
 SHAPES = {
     # name: (dim, points per axis, n, P, Q, renumbering)
@@ -52,3 +52,31 @@ def bytes_per_stencil(n, r, dim=2):
     """Algorithmic HBM bytes: ja (4n), the node's own coordinates (8 dim),
     the weights (8 r n) and the solver status (4)."""
     return 4 * n + 8 * dim + 8 * r * n + 4
+
+
+# Uneven batch on the unit disk: size class by radius for interior nodes,
+# n = 13 (degree 2, r^3) on the boundary.
+DISK_CLASSES = [(13, 2, 3), (21, 2, 3), (35, 3, 5), (56, 4, 7)]
+
+
+def disk_sizes(x, y, flag):
+    r = np.hypot(x, y)
+    n = np.where(r < 1.0 / 3.0, 21, np.where(r < 2.0 / 3.0, 35, 56)).astype(np.int32)
+    n[flag != 0] = 13
+    return n
+
+
+def build_disk(n_interior=2_000_000, n_boundary=8000, seed=5):
+    """Graded unit disk, Hilbert-renumbered, ragged kNN stencils by class."""
+    t0 = time.perf_counter()
+    x, y, flag = graded_disk(n_interior, n_boundary, seed)
+    t1 = time.perf_counter()
+    p = hilbert_order(x, y, 16)
+    x, y, flag = p.permute(x), p.permute(y), p.permute(flag)
+    t2 = time.perf_counter()
+    sizes = disk_sizes(x, y, flag)
+    ia, ja = ragged_stencils(sizes, x, y)
+    t3 = time.perf_counter()
+    return {"x": x, "y": y, "flag": flag, "sizes": sizes, "ia": ia, "ja": ja,
+            "nstencils": x.size,
+            "host_s": {"nodes": t1 - t0, "renumber": t2 - t1, "knn": t3 - t2}}

@@ -137,6 +137,14 @@ RBFA_API int32_t rbfa_knn_stencils(int32_t n, int32_t dim, const double* x, cons
 RBFA_API int32_t rbfa_gen_jittered_grid(int32_t dim, int32_t m, double jitter, uint64_t seed,
                                         double* x, double* y, double* z);
 
+/* Graded unit disk: n_interior nodes on concentric rings whose spacing
+ * falls linearly from hc at the centre to hc/4 at the rim (seeded phase and
+ * +-0.15h jitter per node, a Poisson-disk-like set; hc solved for the
+ * count), then n_boundary equispaced nodes on the unit circle. flag is 0
+ * for interior, 1 for boundary nodes. Interior nodes come first. */
+RBFA_API int32_t rbfa_gen_graded_disk(int32_t n_interior, int32_t n_boundary, uint64_t seed,
+                                      double* x, double* y, int32_t* flag);
+
 /* ======================================================================
  * Device assembly
  * ====================================================================== */
@@ -202,6 +210,35 @@ RBFA_API int32_t rbfa_plan_autotune(rbfa_plan* plan, int32_t nstencils, const do
                                     double* const* w, const double* xc, const double* yc,
                                     const double* zc, int32_t* info, uint8_t* status,
                                     void* stream, int32_t reps, float* ms_out);
+
+/* ---- uneven batches ------------------------------------------------------
+ * Stencils of several sizes in one ragged CSR graph (row s at ia[s], of
+ * length ia[s+1] - ia[s]; the length names the size class). Replaces the
+ * fixed-width launch of assemble_interp_weights (rbf_operators.h:307-310,
+ * implicit ia[s] = s*N) for graphs like the reference's ragged .graph files
+ * (read_graph_csr, rbf_io.h:649-707; rbf_cuda.f90:52-56). Shipped classes
+ * (2-D, FP64, Laplacian/d/dx/d/dy): n/P/Q = 13/2/3, 21/2/3, 35/3/5, 56/4/7.
+ *
+ * bind captures the pipeline into a hipGraph with the given device arrays
+ * (w: host array of 3 device pointers laid out like ja); launch replays
+ * it. Mode 0: one launch per class; mode 1: one persistent launch over
+ * size-bucketed work queues. autotune picks each class's kernel variant and
+ * the faster mode (ms_out[2] = mode 0, mode 1 milliseconds). Calls that
+ * overlap on one batch race on its work-queue counters. */
+typedef struct rbfa_batch rbfa_batch;
+RBFA_API int32_t rbfa_batch_create(int32_t nclasses, const rbfa_plan_desc* classes,
+                                   rbfa_batch** batch);
+RBFA_API int32_t rbfa_batch_destroy(rbfa_batch* batch);
+RBFA_API int32_t rbfa_batch_bind(rbfa_batch* batch, int32_t nstencils, const int32_t* ia_host,
+                                 const double* x, const double* y, const double* z,
+                                 const int32_t* ia, const int32_t* ja, double* const* w,
+                                 int32_t* info, uint8_t* status, void* stream);
+RBFA_API int32_t rbfa_batch_class_count(const rbfa_batch* batch, int32_t cls);
+RBFA_API const char* rbfa_batch_class_variant(const rbfa_batch* batch, int32_t cls);
+RBFA_API int32_t rbfa_batch_set_mode(rbfa_batch* batch, int32_t mode);
+RBFA_API int32_t rbfa_batch_get_mode(const rbfa_batch* batch);
+RBFA_API int32_t rbfa_batch_launch(rbfa_batch* batch, void* stream);
+RBFA_API int32_t rbfa_batch_autotune(rbfa_batch* batch, void* stream, int32_t reps, float* ms_out);
 
 #ifdef __cplusplus
 }

@@ -26,6 +26,7 @@ struct GroupAssembly {
     static constexpr int NC = NT + R;
     static constexpr int NCP = (NC + 1) / 2 * 2;  // row buffer stride (16-byte multiple)
     static constexpr int nrhs = R;
+    static constexpr int nsten = N;  // stencil size
     static_assert(NT > 0 && NT <= 64 * NW, "one system row per lane of the workgroup");
     static_assert(N <= 64 * NW, "one stencil node per lane");
     static_assert(!INTERP, "interpolation operators use the single-wave engine");
@@ -130,8 +131,10 @@ struct GroupAssembly {
         }
     }
 
+    // Stencil s, whose CSR row starts at base (s*N for a fixed-width graph).
     template <class Out>
-    __device__ __forceinline__ static void run(const int s, const double* __restrict__ x,
+    __device__ __forceinline__ static void run(const int s, const size_t base,
+                                               const double* __restrict__ x,
                                                const double* __restrict__ y,
                                                const double* __restrict__ z,
                                                const int* __restrict__ ja, Out out, double* lds,
@@ -140,7 +143,7 @@ struct GroupAssembly {
         const int lane = lane_id();
         const int w = static_cast<int>(threadIdx.x >> 6);
         const int g = w * 64 + lane;  // this lane's row / stencil node
-        const int* ja_row = ja + static_cast<size_t>(s) * N;
+        const int* ja_row = ja + base;
         double* pts = lds + off_pts;
         double* polyT = lds + off_poly;
 
@@ -218,11 +221,11 @@ struct GroupAssembly {
 
         back_from<NT - 1>(row, st, lds + off_x, g);
         if (st.mystep < N) {
-            const size_t base = static_cast<size_t>(s) * N + st.mystep;
+            const size_t at = base + st.mystep;
 #pragma unroll
             for (int c = 0; c < R; ++c) {
                 const double v = row[NT + c] * st.myrcp;
-                out(c)[base] = c == 0 ? v / rho2 : v / rho;
+                out(c)[at] = c == 0 ? v / rho2 : v / rho;
             }
         }
         if (info && g == 0) info[s] = st.info;

@@ -29,6 +29,7 @@ struct WaveMixed {
     using Base = WaveAssembly<DIM, N, P, Q, DIM + 1, false, Tune>;
     using tune = Tune;
     static constexpr int M = Base::M, NT = Base::NT, NC = Base::NC, R = DIM + 1;
+    static constexpr int nsten = N;  // stencil size
     // Base scratch + the iterate x, [R][64] doubles, for the residual.
     static constexpr int lds_doubles = Base::lds_doubles + R * 64;
     static constexpr double kTol = 1e-10;
@@ -68,15 +69,17 @@ struct WaveMixed {
         for (int c = 0; c < R; ++c) r[c] = r[c] * myrcp;
     }
 
+    // Stencil s, whose CSR row starts at base (s*N for a fixed-width graph).
     template <class Out>
-    __device__ __forceinline__ static void run(const int s, const double* __restrict__ x,
+    __device__ __forceinline__ static void run(const int s, const size_t base,
+                               const double* __restrict__ x,
                                const double* __restrict__ y, const double* __restrict__ z,
                                const int* __restrict__ ja, Out out, double* lds,
                                int* __restrict__ info, uint8_t* __restrict__ status,
                                int max_refine) {
 #pragma clang fp contract(off)
         const int lane = lane_id();
-        const int* ja_row = ja + static_cast<size_t>(s) * N;
+        const int* ja_row = ja + base;
         double row[NC];
         double rho2;
         Base::template build_row<double>(s, x, y, z, ja_row, nullptr, nullptr, nullptr, lds, row,
@@ -163,11 +166,11 @@ struct WaveMixed {
         }
         if (mystep < N) {
             const double rho = sqrt(rho2);
-            const size_t base = static_cast<size_t>(s) * N + mystep;
+            const size_t at = base + mystep;
 #pragma unroll
             for (int c = 0; c < R; ++c) {
                 const double v = c == 0 ? xd[c] / rho2 : xd[c] / rho;
-                out(c)[base] = v;
+                out(c)[at] = v;
             }
         }
         if (lane == 0) {

@@ -464,6 +464,7 @@ struct WaveAssembly {
     static constexpr int NT = N + M;
     static constexpr int NC = NT + R;
     static_assert(DIM == 2 || DIM == 3, "2-D or 3-D");
+    static constexpr int nsten = N;  // stencil size
     static_assert(NT <= 64, "one system row per lane of a wave64");
     static_assert(INTERP || R == DIM + 1, "derivative set: Laplacian + one per axis");
     static_assert(Q >= 3 && Q % 2 == 1, "odd PHS exponent >= 3");
@@ -626,15 +627,17 @@ struct WaveAssembly {
         return st.info;
     }
 
+    // Stencil s, whose CSR row starts at base (s*N for a fixed-width graph).
     template <class Out>
-    __device__ __forceinline__ static void run(const int s, const double* __restrict__ x,
+    __device__ __forceinline__ static void run(const int s, const size_t base,
+                               const double* __restrict__ x,
                                const double* __restrict__ y, const double* __restrict__ z,
                                const int* __restrict__ ja, Out out,
                                const double* __restrict__ cx, const double* __restrict__ cy,
                                const double* __restrict__ cz, double* lds,
                                int* __restrict__ info) {
 #pragma clang fp contract(off)
-        const int* ja_row = ja + static_cast<size_t>(s) * N;
+        const int* ja_row = ja + base;
         double row[NC];
         double rho2;
         build_row<double>(s, x, y, z, ja_row, cx, cy, cz, lds, row, rho2);
@@ -643,12 +646,12 @@ struct WaveAssembly {
         const int inf = lu_solve(row, mystep, xo, lds);
         if (mystep < N) {
             const double rho = sqrt(rho2);
-            const size_t base = static_cast<size_t>(s) * N + mystep;
+            const size_t at = base + mystep;
 #pragma unroll
             for (int c = 0; c < R; ++c) {
                 double v = xo[c];
                 if constexpr (!INTERP) v = c == 0 ? v / rho2 : v / rho;
-                out(c)[base] = v;
+                out(c)[at] = v;
             }
         }
         if (info && lane_id() == 0) info[s] = inf;
@@ -720,8 +723,8 @@ __global__ void __launch_bounds__(64) assemble_interp_weights(const int nstencil
     extern __shared__ double rbf_lds_interp[];
     const int s = blockIdx.x;
     if (s >= nstencils) return;
-    Config::engine::run(s, x, y, nullptr, ja, detail::TableOut<T>{A}, xc, yc, nullptr,
-                        rbf_lds_interp, info);
+    Config::engine::run(s, static_cast<size_t>(s) * Config::N, x, y, nullptr, ja,
+                        detail::TableOut<T>{A}, xc, yc, nullptr, rbf_lds_interp, info);
 }
 
 // Laplacian / d/dx / d/dy weights at every node: A[0] Laplacian, A[1]
@@ -737,8 +740,8 @@ __global__ void __launch_bounds__(64) assemble_laplace_weights(const int nstenci
     extern __shared__ double rbf_lds_laplace[];
     const int s = blockIdx.x;
     if (s >= nstencils) return;
-    Config::engine::run(s, x, y, z, ja, detail::TableOut<T>{A}, nullptr, nullptr, nullptr,
-                        rbf_lds_laplace, info);
+    Config::engine::run(s, static_cast<size_t>(s) * Config::N, x, y, z, ja,
+                        detail::TableOut<T>{A}, nullptr, nullptr, nullptr, rbf_lds_laplace, info);
 }
 
 }  // namespace rbf_operators

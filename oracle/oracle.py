@@ -46,6 +46,8 @@ def lib():
         L.oracle_gesv.argtypes = [ctypes.c_int, ctypes.c_int, P, P, P]
         L.oracle_assemble.restype = ctypes.c_int
         L.oracle_assemble.argtypes = [ctypes.c_int] * 6 + [i32, i32] + [P] * 7 + [P, P, ctypes.c_int]
+        L.oracle_assemble_rows.restype = ctypes.c_int
+        L.oracle_assemble_rows.argtypes = [ctypes.c_int] * 4 + [i32] + [P] * 8 + [ctypes.c_int]
         L.oracle_zidx.restype = i64
         L.oracle_zidx.argtypes = [ctypes.c_double, ctypes.c_double, P, ctypes.c_int]
         L.oracle_hidx.restype = i64
@@ -138,6 +140,15 @@ def assemble(dim, n, P, Q, x, y, ja, z=None, op_set=0, centres=None, s0=0, count
     lib().oracle_assemble(dim, n, P, Q, op_set, nrhs, s0, count, _p(x), _p(y), _p(z), _p(ja),
                           _p(cx), _p(cy), _p(cz), tab, _p(info), int(nthreads))
     return w, info
+
+
+def assemble_rows(dim, n, P, Q, rows, ia, x, y, ja, w, info, z=None, nthreads=0):
+    """FP64 weights of the listed stencils (one size class) of a ragged graph,
+    written into the caller's arrays w[op][ia[s] + k] and info[s]."""
+    rows = np.ascontiguousarray(rows, np.int32)
+    tab = (ctypes.c_void_p * len(w))(*[a.ctypes.data for a in w])
+    return lib().oracle_assemble_rows(dim, n, P, Q, rows.size, _p(rows), _p(ia), _p(x), _p(y),
+                                      _p(z), _p(ja), tab, _p(info), int(nthreads))
 
 
 def zidx(px, py, bbox, ndiv):

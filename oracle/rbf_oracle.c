@@ -340,6 +340,46 @@ OR_API int oracle_assemble(int dim, int n, int P, int Q, int op_set, int nrhs, i
     return nbad;
 }
 
+/* Ragged graph: the listed stencils rows[0..count) of size n (all of one
+ * size class), row s at ia[s]; weights to w[op][ia[s] + k], info[s]. */
+OR_API int oracle_assemble_rows(int dim, int n, int P, int Q, int32_t count,
+                                const int32_t* rows, const int32_t* ia, const double* x,
+                                const double* y, const double* z, const int32_t* ja,
+                                double* const* w, int32_t* info, int nthreads) {
+    const int m = oracle_npoly(dim, P), nt = n + m, r = dim + 1;
+    int nbad = 0;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel num_threads(nthreads) reduction(+ : nbad)
+#endif
+    {
+        double* buf = (double*)malloc(sizeof(double) * ((size_t)3 * n + (size_t)nt * nt +
+                                                        (size_t)2 * nt * r));
+        double *xs = buf, *ys = xs + n, *zs = ys + n, *M = zs + n, *B = M + (size_t)nt * nt;
+        double* X = B + (size_t)nt * r;
+#ifdef _OPENMP
+#pragma omp for schedule(static)
+#endif
+        for (int32_t i = 0; i < count; ++i) {
+            const int32_t s = rows[i];
+            const double rho2 = gather_scaled(dim, n, s, x, y, z, &ja[ia[s]], xs, ys, zs);
+            const double rho = sqrt(rho2);
+            oracle_fill(dim, n, P, Q, xs, ys, zs, M);
+            oracle_rhs(dim, n, P, Q, OR_OPS_DIFF, 0, xs, ys, zs, NULL, NULL, NULL, B);
+            const int inf = oracle_gesv(nt, r, M, B, X);
+            if (info) info[s] = inf;
+            if (inf) ++nbad;
+            for (int op = 0; op < r; ++op)
+                for (int k = 0; k < n; ++k) {
+                    const double v = X[k + (size_t)op * nt];
+                    w[op][(size_t)ia[s] + k] = op == 0 ? v / rho2 : v / rho;
+                }
+        }
+        free(buf);
+    }
+    return nbad;
+}
+
 /* ---- curve keys (rbf_ordering.F90:82-226) ---------------------------- */
 
 OR_API int64_t oracle_zidx(double px, double py, const double* bb, int ndiv) {

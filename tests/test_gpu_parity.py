@@ -160,3 +160,59 @@ def test_large_systems_match_oracle(dim, m, n, P, Q):
     e = _relerr(w, wref, n)
     print(f"dim={dim} n={n} nt={plan.nt}: max rel err {e.max():.3e}")
     assert e.max() <= FP64_TOL
+
+
+def _disk_case(n_int=12000, n_bnd=300, seed=21):
+    from rbf_amd import workloads
+    return workloads.build_disk(n_int, n_bnd, seed)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_uneven_batch_matches_oracle_and_replays(mode):
+    """Graded disk with all four size classes in one ragged graph; the
+    captured graph is replayed three times with eager work in between, and
+    every replay is checked against the oracle."""
+    import torch
+    from rbf_amd import workloads
+    W = _disk_case()
+    ia, ja, x, y = W["ia"], W["ja"], W["x"], W["y"]
+    nnz = int(ia[-1])
+    wref = [np.zeros(nnz) for _ in range(3)]
+    iref = np.zeros(x.size, np.int32)
+    for (n, P, Q) in workloads.DISK_CLASSES:
+        rows = np.nonzero(W["sizes"] == n)[0].astype(np.int32)
+        assert rows.size > 0
+        O.assemble_rows(2, n, P, Q, rows, ia, x, y, ja, wref, iref)
+    assert (iref == 0).all()
+    b = R.Batch(workloads.DISK_CLASSES)
+    w = [torch.zeros(nnz, dtype=torch.float64, device="cuda:0") for _ in range(3)]
+    info = torch.full((x.size,), -1, dtype=torch.int32, device="cuda:0")
+    b.bind(ia, _dev(x), _dev(y), _dev(ia), _dev(ja), w, info=info)
+    assert sum(b.class_count(c) for c in range(4)) == x.size
+    b.mode = mode
+    for rep in range(3):
+        for t in w:
+            t.zero_()
+        info.fill_(-1)
+        torch.cuda.synchronize()
+        b.launch()
+        torch.cuda.synchronize()
+        assert (info.cpu().numpy() == 0).all()
+        for a, r in zip(w, wref):
+            g = a.cpu().numpy()
+            err = np.abs(g - r).max() / np.abs(r).max()
+            assert err == 0.0 or err <= FP64_TOL, (rep, err)
+
+
+def test_uneven_batch_autotune():
+    import torch
+    from rbf_amd import workloads
+    W = _disk_case(6000, 200, 22)
+    nnz = int(W["ia"][-1])
+    b = R.Batch(workloads.DISK_CLASSES)
+    w = [torch.zeros(nnz, dtype=torch.float64, device="cuda:0") for _ in range(3)]
+    b.bind(W["ia"], _dev(W["x"]), _dev(W["y"]), _dev(W["ia"]), _dev(W["ja"]), w)
+    ms = b.autotune(reps=2)
+    assert len(ms) == 2 and min(ms) > 0
+    assert b.mode == int(np.argmin(ms))
+    assert all(b.class_variant(c) for c in range(4))

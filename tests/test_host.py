@@ -218,3 +218,37 @@ def test_renumber_matches_reference_build():
     ja = g["ja"].copy()
     O.ref().ref_renumber_stencils(4096, 21, O._p(p.map()), O._p(ja))
     np.testing.assert_array_equal(R.renumber_stencils(g["ja"], 21, p), ja)
+
+
+def test_graded_disk_and_ragged_stencils():
+    from rbf_amd import workloads
+    x, y, flag = R.graded_disk(20000, 500, seed=3)
+    assert x.size == 20500 and (flag[:20000] == 0).all() and (flag[20000:] == 1).all()
+    r = np.hypot(x, y)
+    assert r[:20000].max() < 1.0 and np.allclose(r[20000:], 1.0)
+    x2, y2, _ = R.graded_disk(20000, 500, seed=3)
+    np.testing.assert_array_equal(x, x2)
+    # spacing grades from hc at the centre to hc/4 at the rim
+    ja = R.knn_stencils(2, x, y)[1::2]
+    d = np.hypot(x - x[ja], y - y[ja])
+    inner, outer = r < 0.2, (r > 0.85) & (flag == 0)
+    assert 2.5 < np.median(d[inner]) / np.median(d[outer]) < 6.0
+    sizes = workloads.disk_sizes(x, y, flag)
+    assert set(np.unique(sizes)) == {13, 21, 35, 56}
+    ia, jr = R.ragged_stencils(sizes, x, y)
+    full = R.knn_stencils(56, x, y).reshape(-1, 56)
+    for s in range(0, x.size, 97):
+        np.testing.assert_array_equal(jr[ia[s]:ia[s + 1]], full[s, :sizes[s]])
+        assert jr[ia[s]] == s
+
+
+def test_oracle_ragged_rows_equal_fixed_width():
+    x, y = R.jittered_grid(2, 24, 0.25, seed=4)
+    ja = R.knn_stencils(21, x, y)
+    ia = R.make_row_ptr(x.size, 21)
+    w, info = O.assemble(2, 21, 2, 3, x, y, ja)
+    w2 = [np.zeros(ja.size) for _ in range(3)]
+    info2 = np.zeros(x.size, np.int32)
+    O.assemble_rows(2, 21, 2, 3, np.arange(x.size, dtype=np.int32), ia, x, y, ja, w2, info2)
+    for a, b in zip(w, w2):
+        np.testing.assert_array_equal(a, b)
