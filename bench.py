@@ -1,20 +1,29 @@
 """Benchmark: batched RBF-FD operator assembly on one MI355X.
 
-Workload: 1,048,576 nodes on a jittered 1024x1024 grid of the unit square
-(jitter +-0.25h, seeded), Hilbert-renumbered, n = 35 nearest neighbours,
-PHS r^5 + degree-3 polynomials (45x45 systems), operators Laplacian, d/dx,
-d/dy into CSR (FP64 weights, int32 column indices shared with ja).
+Default workload (the headline metric): 1,048,576 nodes on a jittered
+1024x1024 grid of the unit square (jitter +-0.25h, seeded), Hilbert-
+renumbered, n = 35 nearest neighbours, PHS r^5 + degree-3 polynomials
+(45x45 systems), operators Laplacian, d/dx, d/dy into CSR (FP64 weights,
+int32 column indices shared with ja).
 
-One step = one assembly pass over all 1M stencils, inputs resident in HBM.
+One step = one assembly pass over all stencils, inputs resident in HBM.
 Both forms are timed: the plain FP64 kernel and the mixed form (FP32 LU +
 FP64 refinement, <= 3 steps, FP64 fallback). The headline value is the
 stencils/s of the faster form whose weights stay within 1e-8 (max relative,
 per stencil and operator) of the CPU FP64 oracle over the whole workload;
 both forms, their ratio, the weight errors, the fallback fraction and the
 analytic consistency error are reported beside it. The oracle run doubles
-as the CPU baseline (all host cores, same 1M stencils).
+as the CPU baseline (all host cores, the same stencils).
 
-    python bench.py [--gpus 1] [--steps 20] [--warmup 3]
+Other workloads (--workload), reported in the same format:
+  square_4k_n21    4,096 jittered nodes, n = 21, r^3 + P2 (27x27)
+  square_1m_n56    1M jittered nodes, n = 56, r^7 + P4 (71x71)
+  cube_2m_n80      128^3 jittered unit cube, Morton, n = 80, r^3 + P3 (100x100), 4 operators
+  disk_2m_uneven   graded unit disk, 2,000,000 interior + 8,000 boundary nodes,
+                   n in {13, 21, 35, 56} by size class, one ragged graph,
+                   captured hipGraph (per-class launches or one persistent launch)
+
+    python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload NAME]
 """
 import argparse
 import json
@@ -30,36 +39,60 @@ import numpy as np  # noqa: E402
 
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix alike)
 PEAK_HBM_GBS = 8000.0
+HEADLINE = "square_1m_n35"
+METRIC = "stencils/s assembled, 1M nodes n=35; mixed vs FP64 at <=1e-8 weight rel. err"
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def rel_errors(w_gpu, w_ref, n):
-    """Per stencil and operator: max_k |w - w_ref| / max_k |w_ref|."""
+def rel_errors(w_gpu, w_ref, rows):
+    """Per stencil and operator: max_k |w - w_ref| / max_k |w_ref|; rows is
+    the fixed stencil width n, or the CSR row pointer of a ragged graph."""
     out = []
     for g, r in zip(w_gpu, w_ref):
-        g = g.reshape(-1, n)
-        r = r.reshape(-1, n)
-        out.append(np.abs(g - r).max(1) / np.maximum(np.abs(r).max(1), 1e-300))
+        if isinstance(rows, int):
+            g = g.reshape(-1, rows)
+            r = r.reshape(-1, rows)
+            out.append(np.abs(g - r).max(1) / np.maximum(np.abs(r).max(1), 1e-300))
+        else:
+            d = np.maximum.reduceat(np.abs(g - r), rows[:-1])
+            m = np.maximum.reduceat(np.abs(r), rows[:-1])
+            out.append(d / np.maximum(m, 1e-300))
     return np.stack(out, 1)
 
 
-def consistency(w, ja, x, y, n):
-    """Max relative error of the assembled Laplacian, d/dx, d/dy applied to
-    f = sin(2 pi x) cos(2 pi y), on nodes at least 4 h from the boundary."""
-    J = ja.reshape(-1, n)
-    f = np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)
-    exact = [-8 * np.pi ** 2 * f,
-             2 * np.pi * np.cos(2 * np.pi * x) * np.cos(2 * np.pi * y),
-             -2 * np.pi * np.sin(2 * np.pi * x) * np.sin(2 * np.pi * y)]
-    inner = (x > 0.004) & (x < 0.996) & (y > 0.004) & (y < 0.996)
-    fJ = f[J]
+def test_function(name, pts):
+    """f and its exact Laplacian and gradient at the nodes."""
+    x, y = pts[0], pts[1]
+    if name == "sincos":
+        f = np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)
+        return f, [-8 * np.pi ** 2 * f,
+                   2 * np.pi * np.cos(2 * np.pi * x) * np.cos(2 * np.pi * y),
+                   -2 * np.pi * np.sin(2 * np.pi * x) * np.sin(2 * np.pi * y)]
+    if name == "exp2":    # f = exp(x + y^2)
+        f = np.exp(x + y * y)
+        return f, [(3 + 4 * y * y) * f, f, 2 * y * f]
+    if name == "exp3":    # f = exp(x + y^2 + z/2)
+        z = pts[2]
+        f = np.exp(x + y * y + 0.5 * z)
+        return f, [(3.25 + 4 * y * y) * f, f, 2 * y * f, 0.5 * f]
+    raise ValueError(name)
+
+
+def consistency(w, ja, rows, pts, fname, inner):
+    """Max relative error of the assembled operators applied to f, on the
+    nodes selected by `inner` (away from the boundary)."""
+    f, exact = test_function(fname, pts)
+    fj = f[ja]
     errs = []
     for wi, ex in zip(w, exact):
-        v = np.einsum("ij,ij->i", wi.reshape(-1, n), fJ)
-        errs.append(float(np.abs(v - ex)[inner].max() / np.abs(ex).max()))
+        if isinstance(rows, int):
+            v = np.einsum("ij,ij->i", wi.reshape(-1, rows), fj.reshape(-1, rows))
+        else:
+            v = np.add.reduceat(wi * fj, rows[:-1])
+        errs.append(float(np.abs(v - ex)[inner].max() / np.abs(ex[inner]).max()))
     return errs
 
 
@@ -94,14 +127,151 @@ def read_traffic(kernel_tag):
     return None
 
 
+# name: (consistency test function, default CPU-oracle sample in stencils)
+WORKLOADS = {
+    "square_1m_n35": ("sincos", 1 << 20),
+    "square_4k_n21": ("sincos", 1 << 20),
+    "square_1m_n56": ("exp2", 1 << 20),
+    "cube_2m_n80": ("exp3", 1 << 17),
+    "disk_2m_uneven": ("sincos", 1 << 22),
+}
+
+
+def run_fixed(args, name, R, workloads, torch, dev):
+    W = workloads.build(name, seed=args.seed)
+    dim, n, P, Q, nst = W["dim"], W["n"], W["P"], W["Q"], W["nstencils"]
+    log(f"[{name}] host prep: {W['host_s']}")
+    pts = [W["x"], W["y"]] + ([W["z"]] if dim == 3 else [])
+    dpts = [torch.from_numpy(a).to(dev) for a in pts]
+    ja = torch.from_numpy(W["ja"]).to(dev)
+    nnz = nst * n
+    info = torch.empty(nst, dtype=torch.int32, device=dev)
+    status = torch.empty(nst, dtype=torch.uint8, device=dev)
+    forms = ["fp64", "mixed"] if args.form == "both" else [args.form]
+    res = {}
+    for form in forms:
+        try:
+            plan = R.Plan(dim, n, P, Q, "diff", precision=form, max_refine=3)
+        except R.RbfError as e:
+            log(f"[{form}] {e}")
+            continue
+        w = [torch.empty(nnz, dtype=torch.float64, device=dev) for _ in range(plan.nops)]
+        z = dpts[2] if dim == 3 else None
+
+        def step():
+            plan.assemble(nst, dpts[0], dpts[1], ja, w, z=z, info=info, status=status)
+
+        tune_ms = None
+        if args.variant is not None:
+            plan.variant = args.variant
+        elif not args.no_tune and plan.num_variants > 1:
+            tune_ms = plan.autotune(nst, dpts[0], dpts[1], ja, w, z=z, info=info, status=status,
+                                    reps=2)
+            log(f"[{form}] autotune: " + ", ".join(
+                f"{plan.variant_name(v)}={t:.3f}ms" for v, t in enumerate(tune_ms)))
+        log(f"[{form}] variant {plan.variant_name()}; warmup {args.warmup}, "
+            f"timing {args.steps} steps")
+        ms, wall = hip_timed(step, args.steps, args.warmup)
+        st = status.cpu().numpy()
+        res[form] = {"ms": ms, "wall_s": wall, "singular": int((info != 0).sum().item()),
+                     "fallback": int(((st & R.STAT_FALLBACK) != 0).sum()) if form == "mixed" else 0,
+                     "refine_steps_hist": np.bincount(st & 3, minlength=4).tolist()
+                     if form == "mixed" else None,
+                     "variant": plan.variant_name(),
+                     "autotune_ms": None if tune_ms is None else
+                     {plan.variant_name(v): round(t, 4) for v, t in enumerate(tune_ms)},
+                     "nt": plan.nt, "r": plan.nops, "w": [t.cpu().numpy() for t in w]}
+        log(f"[{form}] {ms:.3f} ms/step, {nst / (ms * 1e-3) / 1e6:.2f} Mstencils/s, "
+            f"singular {res[form]['singular']}, fallback {res[form]['fallback']}")
+        del w
+        torch.cuda.empty_cache()
+    return W, pts, res, n, dim
+
+
+def oracle_fixed(W, res, n, dim, sample):
+    from oracle import oracle as O
+    cores = len(os.sched_getaffinity(0))
+    nst = W["nstencils"]
+    cnt = min(sample, nst)
+    s0 = (nst - cnt) // 2
+    log(f"CPU oracle: {cnt} stencils on {cores} threads")
+    t0 = time.perf_counter()
+    wref, _ = O.assemble(dim, n, W["P"], W["Q"], W["x"], W["y"], W["ja"], z=W["z"], s0=s0,
+                         count=cnt, nthreads=cores)
+    cpu_s = time.perf_counter() - t0
+    cpu = {"value": cnt / cpu_s, "unit": "stencils/s", "cores": cores, "kind": "port",
+           "sample": f"{cnt} contiguous stencils (from {s0}) of the same workload, FP64 oracle "
+                     f"(gather, fill, partial-pivot LU, solve, scatter), {cpu_s:.2f} s"}
+    errs = {}
+    for form, d in res.items():
+        wg = [a[s0 * n:(s0 + cnt) * n] for a in d["w"]]
+        e = rel_errors(wg, wref, n)
+        errs[form] = {"weight_rel_err_max": float(e.max()),
+                      "weight_rel_err_median": float(np.median(e)), "oracle_sample": cnt}
+        log(f"[{form}] weight rel err max {e.max():.3e} median {np.median(e):.3e}")
+    return cpu, errs
+
+
+def run_disk(args, R, workloads, torch, dev):
+    W = workloads.build_disk(seed=args.seed + 3)
+    log(f"[disk] host prep: {W['host_s']}")
+    nst, ia, jah = W["nstencils"], W["ia"], W["ja"]
+    nnz = int(ia[-1])
+    x, y = (torch.from_numpy(a).to(dev) for a in (W["x"], W["y"]))
+    iad, jad = torch.from_numpy(ia).to(dev), torch.from_numpy(jah).to(dev)
+    w = [torch.empty(nnz, dtype=torch.float64, device=dev) for _ in range(3)]
+    info = torch.empty(nst, dtype=torch.int32, device=dev)
+    b = R.Batch(workloads.DISK_CLASSES)
+    b.bind(ia, x, y, iad, jad, w, info=info)
+    counts = {f"n{c[0]}": b.class_count(i) for i, c in enumerate(b.classes)}
+    tune = b.autotune(reps=2) if not args.no_tune else None
+    if tune:
+        log(f"[disk] autotune: per-class launches {tune[0]:.3f} ms, persistent {tune[1]:.3f} ms; "
+            f"mode {b.mode}; variants "
+            + ", ".join(f"n{c[0]}: {b.class_variant(i)}" for i, c in enumerate(b.classes)))
+    ms, wall = hip_timed(b.launch, args.steps, args.warmup)
+    log(f"[disk] {ms:.3f} ms/step, {nst / (ms * 1e-3) / 1e6:.2f} Mstencils/s")
+    res = {"fp64": {"ms": ms, "wall_s": wall, "singular": int((info != 0).sum().item()),
+                    "fallback": 0, "refine_steps_hist": None,
+                    "variant": "mode %d (%s)" % (b.mode, "persistent" if b.mode else "per-class"),
+                    "autotune_ms": None if tune is None else
+                    {"per_class_launches": round(tune[0], 4), "persistent": round(tune[1], 4)},
+                    "class_counts": counts, "w": [t.cpu().numpy() for t in w]}}
+    return W, res
+
+
+def oracle_disk(W, res):
+    from oracle import oracle as O
+    from rbf_amd import workloads
+    cores = len(os.sched_getaffinity(0))
+    ia, ja = W["ia"], W["ja"]
+    nnz = int(ia[-1])
+    wref = [np.zeros(nnz) for _ in range(3)]
+    iref = np.zeros(W["nstencils"], np.int32)
+    t0 = time.perf_counter()
+    for (n, P, Q) in workloads.DISK_CLASSES:
+        rows = np.nonzero(W["sizes"] == n)[0].astype(np.int32)
+        O.assemble_rows(2, n, P, Q, rows, ia, W["x"], W["y"], ja, wref, iref, nthreads=cores)
+    cpu_s = time.perf_counter() - t0
+    cpu = {"value": W["nstencils"] / cpu_s, "unit": "stencils/s", "cores": cores, "kind": "port",
+           "sample": f"all {W['nstencils']} stencils of the workload, FP64 oracle, {cpu_s:.2f} s"}
+    e = rel_errors(res["fp64"]["w"], wref, ia)
+    log(f"[disk] weight rel err max {e.max():.3e} median {np.median(e):.3e}")
+    return cpu, {"fp64": {"weight_rel_err_max": float(e.max()),
+                          "weight_rel_err_median": float(np.median(e)),
+                          "oracle_sample": W["nstencils"]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--seed", type=int, default=2)
-    ap.add_argument("--cpu-sample", type=int, default=1 << 20,
-                    help="stencils in the CPU-baseline / oracle-parity sample (default: all)")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default=HEADLINE)
+    ap.add_argument("--cpu-sample", type=int, default=None,
+                    help="stencils in the CPU-baseline / oracle-parity sample "
+                         "(default: the whole workload, 131072 for cube_2m_n80)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity")
     ap.add_argument("--form", choices=["both", "fp64", "mixed"], default="both")
     ap.add_argument("--no-tune", action="store_true", help="skip the variant autotuner")
@@ -121,63 +291,28 @@ def main():
     from rbf_amd import workloads
 
     dev = torch.device("cuda:0")
-    log("building 1M-node workload (host: jittered grid, Hilbert order, kNN n=35)")
-    W = workloads.build("square_1m_n35", seed=args.seed)
-    n, P, Q, nst = W["n"], W["P"], W["Q"], W["nstencils"]
-    log(f"host prep: {W['host_s']}")
-    x = torch.from_numpy(W["x"]).to(dev)
-    y = torch.from_numpy(W["y"]).to(dev)
-    ja = torch.from_numpy(W["ja"]).to(dev)
-    nnz = nst * n
-    info = torch.empty(nst, dtype=torch.int32, device=dev)
-    status = torch.empty(nst, dtype=torch.uint8, device=dev)
-
-    forms = ["fp64", "mixed"] if args.form == "both" else [args.form]
-    res = {}
-    for form in forms:
-        plan = R.Plan(2, n, P, Q, "diff", precision=form, max_refine=3)
-        nt, r = plan.nt, plan.nops
-        w = [torch.empty(nnz, dtype=torch.float64, device=dev) for _ in range(r)]
-
-        def step():
-            plan.assemble(nst, x, y, ja, w, info=info, status=status)
-
-        tune_ms = None
-        if args.variant is not None:
-            plan.variant = args.variant
-        elif not args.no_tune:
-            tune_ms = plan.autotune(nst, x, y, ja, w, info=info, status=status, reps=2)
-            log(f"[{form}] autotune: " + ", ".join(
-                f"{plan.variant_name(v)}={t:.3f}ms" for v, t in enumerate(tune_ms)))
-        log(f"[{form}] variant {plan.variant_name()}; warmup {args.warmup}, "
-            f"timing {args.steps} steps")
-        ms, wall = hip_timed(step, args.steps, args.warmup)
-        bad = int((info != 0).sum().item())
-        fb = int(((status & R.STAT_FALLBACK) != 0).sum().item()) if form == "mixed" else 0
-        steps_hist = None
-        if form == "mixed":
-            st = (status & R.STAT_STEPS_MASK).to(torch.int64)
-            steps_hist = [int((st == k).sum().item()) for k in range(4)]
-        res[form] = {"ms": ms, "wall_s": wall, "singular": bad, "fallback": fb,
-                     "variant": plan.variant_name(),
-                     "autotune_ms": None if tune_ms is None else
-                     {plan.variant_name(v): round(t, 4) for v, t in enumerate(tune_ms)},
-                     "refine_steps_hist": steps_hist, "nt": nt, "r": r,
-                     "w": [t.cpu().numpy() for t in w]}
-        log(f"[{form}] {ms:.3f} ms/step, {nst / (ms * 1e-3) / 1e6:.1f} Mstencils/s, "
-            f"singular {bad}, fallback {fb}")
-        del w
-        torch.cuda.empty_cache()
-
-    # headline: the faster form whose weights meet the 1e-8 bar (checked
-    # against the oracle below; without the oracle run, the faster form)
-    head = min(res, key=lambda f: res[f]["ms"])
-    nt, r = res[head]["nt"], res[head]["r"]
-    flop = workloads.flops_per_stencil(nt, r)
-    byt = workloads.bytes_per_stencil(n, r)
-    ms = res[head]["ms"]
-    value = nst / (ms * 1e-3)
-    achieved_tflops = flop * nst / (ms * 1e-3) / 1e12
+    name = args.workload
+    fname, default_sample = WORKLOADS[name]
+    sample = args.cpu_sample or default_sample
+    if name == "disk_2m_uneven":
+        W, res = run_disk(args, R, workloads, torch, dev)
+        n_desc, dim, rows = "13/21/35/56", 2, W["ia"]
+        pts = [W["x"], W["y"]]
+        cls_count = [int((W["sizes"] == k[0]).sum()) for k in workloads.DISK_CLASSES]
+        flop = sum(workloads.flops_per_stencil(k[0] + (k[1] + 1) * (k[1] + 2) // 2, 3) * c
+                   for k, c in zip(workloads.DISK_CLASSES, cls_count)) / W["nstencils"]
+        byt = sum(workloads.bytes_per_stencil(k[0], 3) * c
+                  for k, c in zip(workloads.DISK_CLASSES, cls_count)) / W["nstencils"]
+        inner = np.hypot(W["x"], W["y"]) < 0.9
+    else:
+        W, pts, res, n, dim = run_fixed(args, name, R, workloads, torch, dev)
+        n_desc, rows = str(n), n
+        r_any = next(iter(res.values()))
+        flop = workloads.flops_per_stencil(r_any["nt"], r_any["r"])
+        byt = workloads.bytes_per_stencil(n, r_any["r"], dim)
+        h = 1.0 / round(W["nstencils"] ** (1.0 / dim))
+        inner = np.all([(a > 4 * h) & (a < 1 - 4 * h) for a in pts], axis=0)
+    nst = W["nstencils"]
 
     extra = {}
     for form, d in res.items():
@@ -188,42 +323,32 @@ def main():
                        "scatter_gbs": byt * sps / 1e9, "singular": d["singular"],
                        "fallback_fraction": d["fallback"] / nst, "variant": d["variant"],
                        "autotune_ms": d["autotune_ms"]}
-        if d["refine_steps_hist"] is not None:
+        if d.get("refine_steps_hist") is not None:
             extra[form]["refine_steps_hist"] = d["refine_steps_hist"]
+        if d.get("class_counts") is not None:
+            extra[form]["class_counts"] = d["class_counts"]
+        extra[form]["consistency_fn"] = fname
+        extra[form]["consistency_rel_err"] = consistency(d["w"], W["ja"], rows, pts, fname, inner)
     if "fp64" in res and "mixed" in res:
         extra["mixed_over_fp64"] = res["fp64"]["ms"] / res["mixed"]["ms"]
 
     cpu = None
     if not args.no_cpu:
-        from oracle import oracle as O
-        cores = len(os.sched_getaffinity(0))
-        cnt = min(args.cpu_sample, nst)
-        s0 = (nst - cnt) // 2
-        log(f"CPU oracle: {cnt} stencils on {cores} threads")
-        t0 = time.perf_counter()
-        wref, iref = O.assemble(2, n, P, Q, W["x"], W["y"], W["ja"], s0=s0, count=cnt,
-                                nthreads=cores)
-        cpu_s = time.perf_counter() - t0
-        cpu = {"value": cnt / cpu_s, "unit": "stencils/s", "cores": cores, "kind": "port",
-               "sample": f"{cnt} contiguous stencils (from {s0}) of the same 1M workload, "
-                         f"FP64 oracle (gather, fill, partial-pivot LU, solve, scatter), "
-                         f"{cpu_s:.2f} s"}
-        for form, d in res.items():
-            wg = [a[s0 * n:(s0 + cnt) * n] for a in d["w"]]
-            e = rel_errors(wg, wref, n)
-            extra[form]["weight_rel_err_max"] = float(e.max())
-            extra[form]["weight_rel_err_median"] = float(np.median(e))
-            extra[form]["oracle_sample"] = cnt
-            extra[form]["consistency_rel_err"] = consistency(d["w"], W["ja"], W["x"], W["y"], n)
-            log(f"[{form}] weight rel err max {e.max():.3e} median {np.median(e):.3e}")
+        if name == "disk_2m_uneven":
+            cpu, errs = oracle_disk(W, res)
+        else:
+            cpu, errs = oracle_fixed(W, res, n, dim, sample)
+        for form, e in errs.items():
+            extra[form].update(e)
 
-    ok = [f for f in res if extra[f].get("weight_rel_err_max", 0.0) <= 1e-8]
-    if ok and head not in ok:
-        head = min(ok, key=lambda f: res[f]["ms"])
-        nt, r = res[head]["nt"], res[head]["r"]
-        ms = res[head]["ms"]
-        value = nst / (ms * 1e-3)
-        achieved_tflops = flop * nst / (ms * 1e-3) / 1e12
+    # headline: the faster form whose weights meet the 1e-8 bar against the
+    # oracle (without the oracle run, the faster form)
+    ok = [f for f in res if extra[f].get("weight_rel_err_max", 0.0) <= 1e-8] or list(res)
+    head = min(ok, key=lambda f: res[f]["ms"])
+    ms = res[head]["ms"]
+    value = nst / (ms * 1e-3)
+    achieved_tflops = flop * nst / (ms * 1e-3) / 1e12
+
     if args.profile_copy:
         # a plain device copy of known size, for scaling the HBM counters
         src = torch.empty(32 << 20, dtype=torch.float64, device=dev)
@@ -232,9 +357,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log("copy calibration: 3 x 256 MiB device copies")
-    kernel_tag = "assemble_%s_n35_1m" % head
+
+    kernel_tag = "assemble_%s_%s" % (head, "n35_1m" if name == HEADLINE else name)
     line = {
-        "metric": "stencils/s assembled, 1M nodes n=35; mixed vs FP64 at <=1e-8 weight rel. err",
+        "metric": METRIC if name == HEADLINE else f"stencils/s assembled, {name}",
         "value": value,
         "unit": "stencils/s",
         "n_gpus": 1,
@@ -246,10 +372,9 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32+fp64" if head == "mixed" else "f64",
         "form": head,
-        "data": "synthetic (seeded jittered grid, Hilbert order, host kNN)",
-        "config": {"workload": "square_1m_n35: 1024x1024 jittered unit square, Hilbert, "
-                               "n=35, PHS r^5 + P3 (45x45), Laplacian/dx/dy, form=" + head,
-                   "nodes": nst, "n": n, "nt": nt, "ops": r},
+        "data": "synthetic (seeded node generators, curve order, host kNN)",
+        "config": {"workload": name, "nodes": nst, "n": n_desc, "dim": dim,
+                   "ops": dim + 1, "form": head},
         "roofline": {"bound": "mfma", "peak_kind": "fp64 (vector = matrix on MI355X)",
                      "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS,
@@ -258,6 +383,9 @@ def main():
         "forms": extra,
         "host_prep_s": W["host_s"],
     }
+    if name == HEADLINE:
+        line["config"]["workload"] = ("square_1m_n35: 1024x1024 jittered unit square, Hilbert, "
+                                      "n=35, PHS r^5 + P3 (45x45), Laplacian/dx/dy, form=" + head)
     print(json.dumps(line), flush=True)
 
 

@@ -118,6 +118,7 @@ struct Shape {
     int nt, nops;
     const Variant* variants;
     int nvariants;
+    bool needs_ws = false;  // uses the plan's fallback-list workspace
 };
 
 template <int DIM, int N, int P, int Q>
@@ -148,6 +149,17 @@ struct GroupVariants {
         {"workgroup nw=2 bc=lds pv=dpp", &launch_group_diff<DIM, N, P, Q>}};
 };
 template <int DIM, int N, int P, int Q>
+struct GroupMixVariants {
+    static constexpr Variant list[1] = {
+        {"workgroup nw=2 fp32 lu + fp64 refinement", &launch_group_mixed<DIM, N, P, Q>}};
+};
+template <int DIM, int N, int P, int Q>
+constexpr Shape diffgroupmix() {
+    constexpr int M = npoly_dim(DIM, P);
+    return {DIM, N, P, Q, RBFA_OPS_DIFF, 0, RBFA_PREC_MIXED, N + M, DIM + 1,
+            GroupMixVariants<DIM, N, P, Q>::list, 1, true};
+}
+template <int DIM, int N, int P, int Q>
 constexpr Shape diffgroup() {
     constexpr int M = npoly_dim(DIM, P);
     return {DIM, N, P, Q, RBFA_OPS_DIFF, 0, RBFA_PREC_FP64, N + M, DIM + 1,
@@ -177,6 +189,7 @@ const Shape kShapes[] = {
     diffmix<2, 21, 2, 3>(), diffmix<2, 35, 3, 5>(), diffmix<2, 13, 2, 3>(),
     // systems larger than a wavefront: one two-wave workgroup per stencil
     diffgroup<2, 56, 4, 7>(), diffgroup<3, 80, 3, 3>(),
+    diffgroupmix<2, 56, 4, 7>(), diffgroupmix<3, 80, 3, 3>(),
 };
 
 }  // namespace
@@ -185,6 +198,8 @@ struct rbfa_plan {
     rbfa_plan_desc desc;
     Shape shape;
     int variant;
+    int* ws = nullptr;  // fallback workspace (mixed workgroup shapes), grown on demand
+    int64_t ws_cap = 0;
 };
 
 extern "C" {
@@ -221,6 +236,7 @@ RBFA_API int32_t rbfa_plan_create(const rbfa_plan_desc* desc, rbfa_plan** plan) 
 }
 
 RBFA_API int32_t rbfa_plan_destroy(rbfa_plan* plan) {
+    if (plan && plan->ws) (void)hipFree(plan->ws);
     delete plan;
     return RBFA_OK;
 }
@@ -256,6 +272,22 @@ RBFA_API int32_t rbfa_assemble(const rbfa_plan* plan, int32_t nstencils, const d
     a.status = status;
     a.max_refine = plan->desc.max_refine;
     a.stream = static_cast<hipStream_t>(stream);
+    if (sh.needs_ws) {
+        // fallback count + list of up to nstencils entries; grown on demand
+        auto* pl = const_cast<rbfa_plan*>(plan);
+        if (pl->ws_cap < nstencils) {
+            if (pl->ws) {
+                if (hipStreamSynchronize(a.stream) != hipSuccess) return RBFA_ERR_HIP;
+                (void)hipFree(pl->ws);
+                pl->ws = nullptr;
+                pl->ws_cap = 0;
+            }
+            if (hipMalloc(&pl->ws, sizeof(int) * (static_cast<size_t>(nstencils) + 1)) != hipSuccess)
+                return RBFA_ERR_NOMEM;
+            pl->ws_cap = nstencils;
+        }
+        a.ws = pl->ws;
+    }
     return sh.variants[plan->variant].fn(a) == hipSuccess ? RBFA_OK : RBFA_ERR_HIP;
 }
 

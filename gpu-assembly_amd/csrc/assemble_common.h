@@ -20,6 +20,7 @@ struct LaunchArgs {
     int32_t* info;
     uint8_t* status;
     int32_t max_refine;
+    int* ws;  // plan workspace: [0] fallback count, [1..] fallback list
     hipStream_t stream;
 };
 
@@ -38,6 +39,8 @@ inline unsigned xcd_grid(int nblocks) { return 8u * static_cast<unsigned>((nbloc
 // assemble_group.hip, instantiated there for the shipped shapes).
 template <int DIM, int N, int P, int Q>
 hipError_t launch_group_diff(const LaunchArgs& a);
+template <int DIM, int N, int P, int Q>
+hipError_t launch_group_mixed(const LaunchArgs& a);
 
 }  // namespace rbf_amd_internal
 

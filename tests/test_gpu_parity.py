@@ -216,3 +216,24 @@ def test_uneven_batch_autotune():
     assert len(ms) == 2 and min(ms) > 0
     assert b.mode == int(np.argmin(ms))
     assert all(b.class_variant(c) for c in range(4))
+
+
+@pytest.mark.parametrize("dim,m,n,P,Q", [(2, 32, 56, 4, 7), (3, 10, 80, 3, 3)])
+def test_large_systems_mixed(dim, m, n, P, Q):
+    """Mixed precision for systems larger than 64: within 1e-8 of the FP64
+    oracle; stencils that do not converge are re-solved in FP64 (and then
+    match the oracle exactly)."""
+    plan = R.Plan(dim, n, P, Q, "diff", precision="mixed", max_refine=3)
+    pts = R.jittered_grid(dim, m, 0.25, seed=14)
+    ja = R.knn_stencils(n, *pts)
+    z = pts[2] if dim == 3 else None
+    w, info, status = _run(plan, pts[0], pts[1], ja, z=z)
+    wref, iref = O.assemble(dim, n, P, Q, pts[0], pts[1], ja, z=z)
+    assert (info == 0).all()
+    e = _relerr(w, wref, n)
+    fb = (status & 4) != 0
+    print(f"mixed dim={dim} n={n}: max rel err {e.max():.3e}, fallback {fb.mean():.3f}, "
+          f"steps {np.bincount(status & 3, minlength=4)}")
+    assert e.max() <= MIXED_TOL
+    if fb.any():
+        assert e[fb].max() <= FP64_TOL
