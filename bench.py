@@ -22,6 +22,10 @@ Other workloads (--workload), reported in the same format:
   disk_2m_uneven   graded unit disk, 2,000,000 interior + 8,000 boundary nodes,
                    n in {13, 21, 35, 56} by size class, one ragged graph,
                    captured hipGraph (per-class launches or one persistent launch)
+  apply_1m_n35     operator application on the headline workload's operators:
+                   y_c = W_c u for Laplacian/d/dx/d/dy, boundary rows (the
+                   outermost ring of grid cells) set to the identity; one
+                   step = one application, reported in rows/s (HBM-bound)
 
     python bench.py [--gpus 1] [--steps 20] [--warmup 3] [--workload NAME]
 """
@@ -134,132 +138,96 @@ WORKLOADS = {
     "square_1m_n56": ("exp2", 1 << 20),
     "cube_2m_n80": ("exp3", 1 << 17),
     "disk_2m_uneven": ("sincos", 1 << 22),
+    "apply_1m_n35": ("sincos", 1 << 20),
 }
 
 
-def run_fixed(args, name, R, workloads, torch, dev):
-    W = workloads.build(name, seed=args.seed)
-    dim, n, P, Q, nst = W["dim"], W["n"], W["P"], W["Q"], W["nstencils"]
-    log(f"[{name}] host prep: {W['host_s']}")
-    pts = [W["x"], W["y"]] + ([W["z"]] if dim == 3 else [])
-    dpts = [torch.from_numpy(a).to(dev) for a in pts]
-    ja = torch.from_numpy(W["ja"]).to(dev)
-    nnz = nst * n
-    info = torch.empty(nst, dtype=torch.int32, device=dev)
-    status = torch.empty(nst, dtype=torch.uint8, device=dev)
-    forms = ["fp64", "mixed"] if args.form == "both" else [args.form]
-    res = {}
-    for form in forms:
-        try:
-            plan = R.Plan(dim, n, P, Q, "diff", precision=form, max_refine=3)
-        except R.RbfError as e:
-            log(f"[{form}] {e}")
-            continue
-        w = [torch.empty(nnz, dtype=torch.float64, device=dev) for _ in range(plan.nops)]
-        z = dpts[2] if dim == 3 else None
-
-        def step():
-            plan.assemble(nst, dpts[0], dpts[1], ja, w, z=z, info=info, status=status)
-
-        tune_ms = None
-        if args.variant is not None:
-            plan.variant = args.variant
-        elif not args.no_tune and plan.num_variants > 1:
-            tune_ms = plan.autotune(nst, dpts[0], dpts[1], ja, w, z=z, info=info, status=status,
-                                    reps=2)
-            log(f"[{form}] autotune: " + ", ".join(
-                f"{plan.variant_name(v)}={t:.3f}ms" for v, t in enumerate(tune_ms)))
-        log(f"[{form}] variant {plan.variant_name()}; warmup {args.warmup}, "
-            f"timing {args.steps} steps")
-        ms, wall = hip_timed(step, args.steps, args.warmup)
-        st = status.cpu().numpy()
-        res[form] = {"ms": ms, "wall_s": wall, "singular": int((info != 0).sum().item()),
-                     "fallback": int(((st & R.STAT_FALLBACK) != 0).sum()) if form == "mixed" else 0,
-                     "refine_steps_hist": np.bincount(st & 3, minlength=4).tolist()
-                     if form == "mixed" else None,
-                     "variant": plan.variant_name(),
-                     "autotune_ms": None if tune_ms is None else
-                     {plan.variant_name(v): round(t, 4) for v, t in enumerate(tune_ms)},
-                     "nt": plan.nt, "r": plan.nops, "w": [t.cpu().numpy() for t in w]}
-        log(f"[{form}] {ms:.3f} ms/step, {nst / (ms * 1e-3) / 1e6:.2f} Mstencils/s, "
-            f"singular {res[form]['singular']}, fallback {res[form]['fallback']}")
-        del w
-        torch.cuda.empty_cache()
-    return W, pts, res, n, dim
+def copy_calibration(torch, dev):
+    """Three plain 256 MiB device copies: the known-size kernel that scales
+    the HBM byte counters (FETCH_SIZE, WRITE_SIZE) of a profiled run."""
+    src = torch.empty(32 << 20, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log("copy calibration: 3 x 256 MiB device copies")
 
 
-def oracle_fixed(W, res, n, dim, sample):
+def run_apply(args, R, workloads, torch, dev):
+    """Operator application y_c = W_c u (+ boundary identity) over the
+    assembled headline operators; HBM-bound, roofline in GB/s. Timed: the
+    planned form (blocked layout built once) and the CSR form as it is."""
     from oracle import oracle as O
-    cores = len(os.sched_getaffinity(0))
-    nst = W["nstencils"]
-    cnt = min(sample, nst)
-    s0 = (nst - cnt) // 2
-    log(f"CPU oracle: {cnt} stencils on {cores} threads")
-    t0 = time.perf_counter()
-    wref, _ = O.assemble(dim, n, W["P"], W["Q"], W["x"], W["y"], W["ja"], z=W["z"], s0=s0,
-                         count=cnt, nthreads=cores)
-    cpu_s = time.perf_counter() - t0
-    cpu = {"value": cnt / cpu_s, "unit": "stencils/s", "cores": cores, "kind": "port",
-           "sample": f"{cnt} contiguous stencils (from {s0}) of the same workload, FP64 oracle "
-                     f"(gather, fill, partial-pivot LU, solve, scatter), {cpu_s:.2f} s"}
-    errs = {}
-    for form, d in res.items():
-        wg = [a[s0 * n:(s0 + cnt) * n] for a in d["w"]]
-        e = rel_errors(wg, wref, n)
-        errs[form] = {"weight_rel_err_max": float(e.max()),
-                      "weight_rel_err_median": float(np.median(e)), "oracle_sample": cnt}
-        log(f"[{form}] weight rel err max {e.max():.3e} median {np.median(e):.3e}")
-    return cpu, errs
-
-
-def run_disk(args, R, workloads, torch, dev):
-    W = workloads.build_disk(seed=args.seed + 3)
-    log(f"[disk] host prep: {W['host_s']}")
-    nst, ia, jah = W["nstencils"], W["ia"], W["ja"]
-    nnz = int(ia[-1])
-    x, y = (torch.from_numpy(a).to(dev) for a in (W["x"], W["y"]))
-    iad, jad = torch.from_numpy(ia).to(dev), torch.from_numpy(jah).to(dev)
-    w = [torch.empty(nnz, dtype=torch.float64, device=dev) for _ in range(3)]
+    W = workloads.build("square_1m_n35", seed=args.seed)
+    nst, n = W["nstencils"], W["n"]
+    x, y, jah = W["x"], W["y"], W["ja"]
+    m = round(nst ** 0.5)
+    h = 1.0 / m
+    bnd_h = np.nonzero((x < h) | (x > 1 - h) | (y < h) | (y > 1 - h))[0].astype(np.int32)
+    xd, yd, ja = (torch.from_numpy(a).to(dev) for a in (x, y, jah))
+    plan = R.Plan(2, n, 3, 5, "diff")
+    w = [torch.empty(nst * n, dtype=torch.float64, device=dev) for _ in range(3)]
     info = torch.empty(nst, dtype=torch.int32, device=dev)
-    b = R.Batch(workloads.DISK_CLASSES)
-    b.bind(ia, x, y, iad, jad, w, info=info)
-    counts = {f"n{c[0]}": b.class_count(i) for i, c in enumerate(b.classes)}
-    tune = b.autotune(reps=2) if not args.no_tune else None
-    if tune:
-        log(f"[disk] autotune: per-class launches {tune[0]:.3f} ms, persistent {tune[1]:.3f} ms; "
-            f"mode {b.mode}; variants "
-            + ", ".join(f"n{c[0]}: {b.class_variant(i)}" for i, c in enumerate(b.classes)))
-    ms, wall = hip_timed(b.launch, args.steps, args.warmup)
-    log(f"[disk] {ms:.3f} ms/step, {nst / (ms * 1e-3) / 1e6:.2f} Mstencils/s")
-    res = {"fp64": {"ms": ms, "wall_s": wall, "singular": int((info != 0).sum().item()),
-                    "fallback": 0, "refine_steps_hist": None,
-                    "variant": "mode %d (%s)" % (b.mode, "persistent" if b.mode else "per-class"),
-                    "autotune_ms": None if tune is None else
-                    {"per_class_launches": round(tune[0], 4), "persistent": round(tune[1], 4)},
-                    "class_counts": counts, "w": [t.cpu().numpy() for t in w]}}
-    return W, res
-
-
-def oracle_disk(W, res):
-    from oracle import oracle as O
-    from rbf_amd import workloads
-    cores = len(os.sched_getaffinity(0))
-    ia, ja = W["ia"], W["ja"]
-    nnz = int(ia[-1])
-    wref = [np.zeros(nnz) for _ in range(3)]
-    iref = np.zeros(W["nstencils"], np.int32)
+    plan.assemble(nst, xd, yd, ja, w, info=info)
+    f, exact = test_function("sincos", [x, y])
+    u = torch.from_numpy(f).to(dev)
+    bnd = torch.from_numpy(bnd_h).to(dev)
     t0 = time.perf_counter()
-    for (n, P, Q) in workloads.DISK_CLASSES:
-        rows = np.nonzero(W["sizes"] == n)[0].astype(np.int32)
-        O.assemble_rows(2, n, P, Q, rows, ia, W["x"], W["y"], ja, wref, iref, nthreads=cores)
-    cpu_s = time.perf_counter() - t0
-    cpu = {"value": W["nstencils"] / cpu_s, "unit": "stencils/s", "cores": cores, "kind": "port",
-           "sample": f"all {W['nstencils']} stencils of the workload, FP64 oracle, {cpu_s:.2f} s"}
-    e = rel_errors(res["fp64"]["w"], wref, ia)
-    log(f"[disk] weight rel err max {e.max():.3e} median {np.median(e):.3e}")
-    return cpu, {"fp64": {"weight_rel_err_max": float(e.max()),
-                          "weight_rel_err_median": float(np.median(e)),
-                          "oracle_sample": W["nstencils"]}}
+    ap = R.ApplyPlan(jah, k=n, bnd=bnd_h)
+    setup_s = time.perf_counter() - t0
+    st = ap.stats()
+    log(f"[apply] plan: {setup_s:.3f} s, {st}")
+    out = [torch.empty(nst, dtype=torch.float64, device=dev) for _ in range(3)]
+    out_csr = [torch.empty(nst, dtype=torch.float64, device=dev) for _ in range(3)]
+
+    def step():
+        ap.run(w, u, out)
+
+    def step_csr():
+        R.apply(ja, w, u, out_csr, k=n, bnd=bnd)
+
+    log(f"[apply] {nst} rows x {n}, 3 operators, {bnd_h.size} identity rows; warmup "
+        f"{args.warmup}, timing {args.steps} steps")
+    ms, wall = hip_timed(step, args.steps, args.warmup)
+    ms_csr, _ = hip_timed(step_csr, args.steps, args.warmup)
+    yg = [t.cpu().numpy() for t in out]
+    yc = [t.cpu().numpy() for t in out_csr]
+    wh = [t.cpu().numpy() for t in w]
+    # algorithmic bytes of the planned form: weights, 16-bit local indices,
+    # distinct columns (index + gathered value), identity flags, results
+    row_bytes = (8 * 3 * n + 2 * n + 12 * st["distinct_total"] / nst + 1 + 8 * 3)
+    row_bytes_csr = 4 * n + 8 * 3 * n + 8 + 8 * 3
+    gbs = row_bytes * nst / (ms * 1e-3) / 1e9
+    log(f"[apply] planned {ms * 1e3:.1f} us/step ({gbs:.0f} GB/s); CSR form "
+        f"{ms_csr * 1e3:.1f} us/step ({row_bytes_csr * nst / (ms_csr * 1e-3) / 1e9:.0f} GB/s)")
+    res = {"ms": ms, "wall_s": wall, "gbs": gbs, "row_bytes": row_bytes,
+           "identity_rows": int(bnd_h.size), "plan": st, "plan_setup_s": setup_s,
+           "csr_form": {"ms_per_step": round(ms_csr, 5), "bytes_per_row": row_bytes_csr,
+                        "gbs": row_bytes_csr * nst / (ms_csr * 1e-3) / 1e9}}
+    inner = np.ones(nst, bool)
+    inner[bnd_h] = False
+    inner &= (x > 4 * h) & (x < 1 - 4 * h) & (y > 4 * h) & (y < 1 - 4 * h)
+    res["consistency_rel_err"] = [float(np.abs(a - e)[inner].max() / np.abs(e[inner]).max())
+                                  for a, e in zip(yg, exact)]
+    cpu = None
+    if not args.no_cpu:
+        ia = np.arange(0, nst * n + 1, n, dtype=np.int64)
+        t0 = time.perf_counter()
+        refs = [O.apply(ia, jah, wc, f, bnd_h) for wc in wh]
+        cpu_s = time.perf_counter() - t0
+        cpu = {"value": nst / cpu_s, "unit": "rows/s (3 operators)", "cores": 1, "kind": "port",
+               "sample": f"all {nst} rows, 3 operators, numpy oracle (column-ordered row sums), "
+                         f"{cpu_s:.2f} s"}
+        res["planned_bit_identical_rows"] = int(sum((a == yr).sum() for a, (yr, _) in
+                                                    zip(yg, refs)))
+        res["rows_checked"] = 3 * nst
+        bound = [float((np.abs(a - yr) / np.maximum(2 * n * np.finfo(float).eps * sc,
+                                                    1e-300)).max())
+                 for a, (yr, sc) in zip(yc, refs)]
+        res["csr_form"]["err_over_rounding_bound_max"] = max(bound)
+        log(f"[apply] planned: {res['planned_bit_identical_rows']} / {3 * nst} rows bit-identical "
+            f"to the oracle; CSR form max |y - y_oracle| / (2 n eps sum|w u|) = {max(bound):.3f}")
+    return W, res, cpu
 
 
 def main():
@@ -294,6 +262,36 @@ def main():
     name = args.workload
     fname, default_sample = WORKLOADS[name]
     sample = args.cpu_sample or default_sample
+    if name == "apply_1m_n35":
+        W, res, cpu = run_apply(args, R, workloads, torch, dev)
+        if args.profile_copy:
+            copy_calibration(torch, dev)
+        ms = res["ms"]
+        value = W["nstencils"] / (ms * 1e-3)
+        line = {
+            "metric": "rows/s of operator application (Laplacian, d/dx, d/dy), 1M nodes n=35",
+            "value": value, "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms, 5), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+            "data": "synthetic (seeded node generators, curve order, host kNN)",
+            "config": {"workload": "apply_1m_n35: y_c = W_c u over the square_1m_n35 operators "
+                                   "(3 x 36.7M nonzeros) + boundary identity rows, planned form",
+                       "nodes": W["nstencils"], "n": 35, "ops": 3},
+            "roofline": {"bound": "hbm", "achieved": res["gbs"], "peak": PEAK_HBM_GBS,
+                         "unit": "GB/s", "frac": res["gbs"] / PEAK_HBM_GBS,
+                         "bytes_per_row": res["row_bytes"],
+                         "traffic": read_traffic("apply_planned_n35_1m")},
+            "cpu_baseline": cpu,
+            "parity": {"planned_bit_identical_rows": res.get("planned_bit_identical_rows"),
+                       "rows_checked": res.get("rows_checked"),
+                       "consistency_fn": fname,
+                       "consistency_rel_err": res["consistency_rel_err"]},
+            "plan": res["plan"], "plan_setup_s": res["plan_setup_s"],
+            "csr_form": res["csr_form"],
+            "host_prep_s": W["host_s"],
+        }
+        print(json.dumps(line), flush=True)
+        return
     if name == "disk_2m_uneven":
         W, res = run_disk(args, R, workloads, torch, dev)
         n_desc, dim, rows = "13/21/35/56", 2, W["ia"]
@@ -350,13 +348,7 @@ def main():
     achieved_tflops = flop * nst / (ms * 1e-3) / 1e12
 
     if args.profile_copy:
-        # a plain device copy of known size, for scaling the HBM counters
-        src = torch.empty(32 << 20, dtype=torch.float64, device=dev)
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        log("copy calibration: 3 x 256 MiB device copies")
+        copy_calibration(torch, dev)
 
     kernel_tag = "assemble_%s_%s" % (head, "n35_1m" if name == HEADLINE else name)
     line = {

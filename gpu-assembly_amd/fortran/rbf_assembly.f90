@@ -31,7 +31,7 @@ module rbf_assembly
   public :: rbfa_morton_keys, rbfa_hilbert_keys, rbfa_morton_keys_3d, rbfa_order_by_keys
   public :: rbfa_renumber_stencils, rbfa_make_row_ptr, rbfa_knn_stencils
   public :: rbfa_gen_jittered_grid, rbfa_plan_create, rbfa_plan_destroy, rbfa_plan_nops
-  public :: rbfa_plan_system_size, rbfa_assemble
+  public :: rbfa_plan_system_size, rbfa_assemble, rbfa_apply
 
   interface
     integer(c_int32_t) function rbfa_morton_keys(n, x, y, bbox, ndiv, keys) bind(c)
@@ -122,6 +122,15 @@ module rbf_assembly
       type(c_ptr), value :: plan, x, y, z, ja, xc, yc, zc, info, status, stream
       integer(c_int32_t), value :: nstencils
       type(c_ptr), intent(in) :: w(*)
+    end function
+
+    ! y_c = W_c u, then y_c(bnd) = u(bnd): the mv_op of src/main.cpp:59-66
+    integer(c_int32_t) function rbfa_apply(nrows, k, ia, ja, nops, w, u, y, nbnd, bnd, &
+                                           stream) bind(c)
+      import :: c_int32_t, c_ptr
+      integer(c_int32_t), value :: nrows, k, nops, nbnd
+      type(c_ptr), value :: ia, ja, u, bnd, stream
+      type(c_ptr), intent(in) :: w(*), y(*)
     end function
   end interface
 

@@ -94,6 +94,11 @@ def lib():
         "rbfa_batch_get_mode": (i32, [P]),
         "rbfa_batch_launch": (i32, [P, P]),
         "rbfa_batch_autotune": (i32, [P, P, i32, P]),
+        "rbfa_apply": (i32, [i32, i32, P, P, i32, P, P, P, i32, P, P]),
+        "rbfa_apply_plan_create": (i32, [i32, i32, i32, P, P, i32, P, P]),
+        "rbfa_apply_plan_destroy": (i32, [P]),
+        "rbfa_apply_plan_stats": (i32, [P, P]),
+        "rbfa_apply_plan_run": (i32, [P, i32, P, P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -401,6 +406,94 @@ class Batch:
         _check(lib().rbfa_batch_autotune(self._h, ctypes.c_void_p(stream), int(reps), ms),
                "rbfa_batch_autotune")
         return list(ms)
+
+
+def apply(ja, w, u, y, k=None, ia=None, bnd=None, stream=None):
+    """Operator application (the mv_op of the reference's main.cpp:59-66):
+    y[c] = W[c] u for each assembled operator c, then y[c][b] = u[b] for the
+    rows b in bnd. ja, w[*], u, y[*], ia, bnd are torch CUDA tensors
+    (int32 / float64); give k for fixed-width rows or ia for ragged rows."""
+    import torch
+
+    def dp(t):
+        return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+    if (k is None) == (ia is None):
+        raise ValueError("give exactly one of k (fixed width) or ia (row pointers)")
+    if len(w) != len(y) or not 1 <= len(w) <= 4:
+        raise ValueError("w and y must hold the same number (1..4) of operators")
+    nrows = y[0].numel()
+    nnz = int(ia[-1].item()) if ia is not None else nrows * int(k)
+    for t in (ja, u, *w, *y) + ((ia,) if ia is not None else ()) + ((bnd,) if bnd is not None else ()):
+        if not t.is_cuda or not t.is_contiguous():
+            raise ValueError("device tensors must be contiguous CUDA tensors")
+    if ja.dtype != torch.int32 or ja.numel() < nnz or any(
+            t.dtype != torch.float64 or t.numel() < nnz for t in w):
+        raise ValueError("ja must be int32 and w float64 with nnz entries")
+    if u.dtype != torch.float64 or any(t.dtype != torch.float64 or t.numel() != nrows for t in y):
+        raise ValueError("u and y must be float64, y with one entry per row")
+    if ia is not None and (ia.dtype != torch.int32 or ia.numel() != nrows + 1):
+        raise ValueError("ia must be int32 with nrows + 1 entries")
+    if bnd is not None and bnd.dtype != torch.int32:
+        raise ValueError("bnd must be int32")
+    wt = (ctypes.c_void_p * len(w))(*[t.data_ptr() for t in w])
+    yt = (ctypes.c_void_p * len(y))(*[t.data_ptr() for t in y])
+    if stream is None:
+        stream = torch.cuda.current_stream().cuda_stream
+    nb = 0 if bnd is None else bnd.numel()
+    _check(lib().rbfa_apply(nrows, 0 if k is None else int(k), dp(ia), dp(ja), len(w), wt, dp(u),
+                            yt, nb, dp(bnd) if nb else None, ctypes.c_void_p(stream)),
+           "rbfa_apply")
+
+
+class ApplyPlan:
+    """Planned operator application over one CSR structure (host arrays:
+    ja, and ia or a fixed width k; bnd = identity rows): run(w, u, y) sets
+    y[c] = W[c] u, equal bit for bit to the sequential CSR product."""
+
+    def __init__(self, ja, k=None, ia=None, bnd=None, ncols=None):
+        if (k is None) == (ia is None):
+            raise ValueError("give exactly one of k (fixed width) or ia (row pointers)")
+        ja = _i32(ja)
+        ia = None if ia is None else _i32(ia)
+        nrows = ia.size - 1 if ia is not None else ja.size // int(k)
+        bnd = None if bnd is None else _i32(bnd)
+        self._h = ctypes.c_void_p()
+        _check(lib().rbfa_apply_plan_create(nrows, int(ncols or nrows), 0 if k is None else int(k),
+                                            None if ia is None else _ptr(ia), _ptr(ja),
+                                            0 if bnd is None else bnd.size,
+                                            None if bnd is None else _ptr(bnd),
+                                            ctypes.byref(self._h)), "rbfa_apply_plan_create")
+        self.nrows = nrows
+        self.nnz = int(ia[-1]) if ia is not None else nrows * int(k)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.rbfa_apply_plan_destroy(h)
+            self._h = None
+
+    def stats(self):
+        out = np.zeros(5, np.int64)
+        _check(lib().rbfa_apply_plan_stats(self._h, _ptr(out)), "rbfa_apply_plan_stats")
+        return dict(zip(["blocks", "max_distinct", "distinct_total", "nnz", "device_bytes"],
+                        out.tolist()))
+
+    def run(self, w, u, y, stream=None):
+        import torch
+        if len(w) != len(y) or not 1 <= len(w) <= 4:
+            raise ValueError("w and y must hold the same number (1..4) of operators")
+        for t in (u, *w, *y):
+            if not t.is_cuda or not t.is_contiguous() or t.dtype != torch.float64:
+                raise ValueError("w, u, y must be contiguous float64 CUDA tensors")
+        if any(t.numel() < self.nnz for t in w) or any(t.numel() != self.nrows for t in y):
+            raise ValueError("w needs nnz entries, y one entry per row")
+        wt = (ctypes.c_void_p * len(w))(*[t.data_ptr() for t in w])
+        yt = (ctypes.c_void_p * len(y))(*[t.data_ptr() for t in y])
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        _check(lib().rbfa_apply_plan_run(self._h, len(w), wt, ctypes.c_void_p(u.data_ptr()), yt,
+                                         ctypes.c_void_p(stream)), "rbfa_apply_plan_run")
 
 
 class Plan:

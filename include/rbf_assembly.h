@@ -240,6 +240,40 @@ RBFA_API int32_t rbfa_batch_get_mode(const rbfa_batch* batch);
 RBFA_API int32_t rbfa_batch_launch(rbfa_batch* batch, void* stream);
 RBFA_API int32_t rbfa_batch_autotune(rbfa_batch* batch, void* stream, int32_t reps, float* ms_out);
 
+/* ------------------------------------------------------------ operator application
+ * Replaces the mv_op of src/main.cpp:59-66 (A.spmv(x, y), then
+ * y[bnd[k]] = x[bnd[k]]): y_c = W_c u for nops (1..4) assembled operators
+ * sharing one CSR structure (ia: nrows+1 device row pointers, or NULL for
+ * fixed width k), then the identity on the nbnd listed rows of every y_c.
+ * All pointers are device pointers except the host arrays w and y of nops
+ * device pointers each. y_c must not alias u. Row sums are reassociated
+ * (eight partial sums per row), so results agree with a sequential sum to
+ * within k * eps * sum_j |w_j u_j|. */
+RBFA_API int32_t rbfa_apply(int32_t nrows, int32_t k, const int32_t* ia, const int32_t* ja,
+                            int32_t nops, double* const* w, const double* u, double* const* y,
+                            int32_t nbnd, const int32_t* bnd, void* stream);
+
+/* Planned operator application: the same mv_op (src/main.cpp:59-66) for a
+ * structure applied many times (the reference applies it once per solver
+ * iteration). create takes HOST arrays (ia: nrows+1 row pointers or NULL
+ * for fixed width k; ja: column indices in [0, ncols); bnd: the nbnd
+ * identity rows) and builds, on the current device, a blocked copy of the
+ * structure: consecutive row blocks (<= 256 rows, <= 2048 nonzeros), each
+ * with its sorted distinct columns and a 16-bit local column index per
+ * nonzero. run computes y_c = W_c u (identity on the bnd rows) for nops
+ * (1..4) weight arrays in the original CSR entry order; every row is summed
+ * left to right, so y equals the sequential CSR product bit for bit.
+ * stats: out[0..4] = blocks, most distinct columns in a block, total
+ * distinct-column entries, nonzeros, device bytes held by the plan. */
+typedef struct rbfa_apply_plan rbfa_apply_plan;
+RBFA_API int32_t rbfa_apply_plan_create(int32_t nrows, int32_t ncols, int32_t k,
+                                        const int32_t* ia, const int32_t* ja, int32_t nbnd,
+                                        const int32_t* bnd, rbfa_apply_plan** plan);
+RBFA_API int32_t rbfa_apply_plan_destroy(rbfa_apply_plan* plan);
+RBFA_API int32_t rbfa_apply_plan_stats(const rbfa_apply_plan* plan, int64_t* out);
+RBFA_API int32_t rbfa_apply_plan_run(const rbfa_apply_plan* plan, int32_t nops, double* const* w,
+                                     const double* u, double* const* y, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -171,3 +171,26 @@ def knn_brute(k, x, y, z=None):
 
 def max_threads():
     return lib().oracle_max_threads()
+
+
+def apply(ia, ja, w, u, bnd=None):
+    """The reference's mv_op (src/main.cpp:59-66): CSR product y = W u with
+    each row summed left to right, then y[b] = u[b] on the boundary rows b.
+    Also returns sum_j |w_j u_j| per row, the scale of the rounding bound
+    for any reassociated sum. Numpy; w is one operator's value array."""
+    ia = np.asarray(ia, np.int64)
+    rows = np.repeat(np.arange(ia.size - 1), np.diff(ia))
+    pos = np.arange(ia[-1]) - ia[rows]
+    width = int(np.diff(ia).max()) if ia.size > 1 else 0
+    prod = np.asarray(w[:ia[-1]], float) * np.asarray(u, float)[np.asarray(ja[:ia[-1]])]
+    y = np.zeros(ia.size - 1)
+    for j in range(width):                 # column j of every row, in order
+        sel = pos == j
+        y[rows[sel]] += prod[sel]
+    scale = np.zeros(ia.size - 1)
+    np.add.at(scale, rows, np.abs(prod))
+    if bnd is not None:
+        b = np.asarray(bnd, np.int64)
+        y[b] = np.asarray(u, float)[b]
+        scale[b] = np.abs(np.asarray(u, float)[b])
+    return y, scale

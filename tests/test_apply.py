@@ -1,0 +1,161 @@
+"""Operator application (the mv_op of the reference's main.cpp:59-66) on the
+GPU, through the C ABI, against the oracle's left-to-right CSR product.
+
+The device sums each row as eight interleaved partial sums, so the bar is
+the rounding bound of a reassociated sum: |y - y_ref| <= 2 k eps S per row,
+S = sum_j |w_j u_j| (for identity rows S = |u_b| and the bar is exact).
+"""
+import numpy as np
+import pytest
+
+import rbf_amd as R
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+EPS = np.finfo(float).eps
+
+
+def _dev(a):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda:0")
+
+
+def _assemble(plan, x, y, ja):
+    import torch
+    nst = x.size
+    w = [torch.empty(ja.size, dtype=torch.float64, device="cuda:0") for _ in range(plan.nops)]
+    info = torch.full((nst,), -1, dtype=torch.int32, device="cuda:0")
+    plan.assemble(nst, _dev(x), _dev(y), _dev(ja), w, info=info)
+    torch.cuda.synchronize()
+    assert (info.cpu().numpy() == 0).all()
+    return w
+
+
+def _apply(ja, w, u, k=None, ia=None, bnd=None):
+    import torch
+    nrows = (ia.size - 1) if ia is not None else ja.size // k
+    y = [torch.full((nrows,), np.nan, dtype=torch.float64, device="cuda:0") for _ in w]
+    R.apply(_dev(ja), w, _dev(u), y, k=k, ia=None if ia is None else _dev(ia),
+            bnd=None if bnd is None else _dev(bnd))
+    torch.cuda.synchronize()
+    return [t.cpu().numpy() for t in y]
+
+
+def _check(yg, ia, ja, w, u, bnd=None):
+    kmax = int(np.diff(ia).max())
+    for a, wc in zip(yg, w):
+        yr, scale = O.apply(ia, ja, wc.cpu().numpy(), u, bnd)
+        err = np.abs(a - yr)
+        assert np.isfinite(a).all()
+        assert (err <= 2 * kmax * EPS * scale).all(), err.max()
+
+
+@pytest.mark.parametrize("n,P,Q", [(21, 2, 3), (35, 3, 5)])
+def test_apply_matches_oracle_fixed_width(n, P, Q):
+    x, y = R.jittered_grid(2, 64, 0.25, seed=31)
+    p = R.hilbert_order(x, y)
+    x, y = p.permute(x), p.permute(y)
+    ja = R.knn_stencils(n, x, y)
+    w = _assemble(R.Plan(2, n, P, Q, "diff"), x, y, ja)
+    u = np.sin(3 * x) * np.cos(2 * y)
+    yg = _apply(ja, w, u, k=n)
+    _check(yg, R.make_row_ptr(x.size, n), ja, w, u)
+    # one operator alone gives the same values as inside the triple
+    y1 = _apply(ja, w[:1], u, k=n)
+    np.testing.assert_array_equal(y1[0], yg[0])
+
+
+def test_apply_ragged_with_boundary_identity():
+    """Graded disk, four size classes in one ragged graph, boundary rows
+    replaced by the identity (y[b] = u[b])."""
+    import torch
+    from rbf_amd import workloads
+    W = workloads.build_disk(9000, 300, seed=32)
+    ia, ja, x, y = W["ia"], W["ja"], W["x"], W["y"]
+    b = R.Batch(workloads.DISK_CLASSES)
+    w = [torch.zeros(int(ia[-1]), dtype=torch.float64, device="cuda:0") for _ in range(3)]
+    b.bind(ia, _dev(x), _dev(y), _dev(ia), _dev(ja), w)
+    b.launch()
+    torch.cuda.synchronize()
+    bnd = np.nonzero(W["flag"])[0].astype(np.int32)
+    u = np.exp(x) * np.sin(y) + 0.5
+    yg = _apply(ja, w, u, ia=ia, bnd=bnd)
+    _check(yg, ia, ja, w, u, bnd)
+    for a in yg:
+        np.testing.assert_array_equal(a[bnd], u[bnd])
+
+
+def test_apply_reproduces_quadratics_at_benchmark_size():
+    """The consistency check on the device at the benchmark size (1M nodes,
+    n = 35, degree 3): the weights reproduce every polynomial of degree <= 3,
+    so for f = x^2 + y^2 the Laplacian is 4 and d/dx, d/dy are 2x, 2y, to
+    within the rounding of the row sum (a closed form, independent of the
+    oracle)."""
+    from rbf_amd import workloads
+    W = workloads.build("square_1m_n35")
+    x, y, ja = W["x"], W["y"], W["ja"]
+    w = _assemble(R.Plan(2, 35, 3, 5, "diff"), x, y, ja)
+    # shift to the cloud's centre so f stays O(1) and the bound is tight
+    xs, ys = x - 0.5, y - 0.5
+    f = xs * xs + ys * ys
+    yg = _apply(ja, w, f, k=35)
+    want = [np.full(x.size, 4.0), 2 * xs, 2 * ys]
+    for c, (a, e) in enumerate(zip(yg, want)):
+        scale = np.abs(w[c].cpu().numpy().reshape(-1, 35)).sum(1) * np.abs(f).max()
+        rel = np.abs(a - e) / scale
+        print(f"op {c}: max |err|/(sum|w| max|f|) = {rel.max():.3e}, max |err| = "
+              f"{np.abs(a - e).max():.3e}")
+        assert rel.max() <= 1e-12
+
+
+def _planned(plan, w, u):
+    import torch
+    y = [torch.full((plan.nrows,), np.nan, dtype=torch.float64, device="cuda:0") for _ in w]
+    plan.run(w, _dev(u), y)
+    torch.cuda.synchronize()
+    return [t.cpu().numpy() for t in y]
+
+
+@pytest.mark.parametrize("n,P,Q", [(21, 2, 3), (35, 3, 5)])
+def test_planned_apply_is_bit_identical_to_sequential_product(n, P, Q):
+    x, y = R.jittered_grid(2, 80, 0.25, seed=33)
+    p = R.hilbert_order(x, y)
+    x, y = p.permute(x), p.permute(y)
+    ja = R.knn_stencils(n, x, y)
+    w = _assemble(R.Plan(2, n, P, Q, "diff"), x, y, ja)
+    u = np.cos(5 * x) + y * y
+    plan = R.ApplyPlan(ja, k=n)
+    st = plan.stats()
+    assert st["nnz"] == ja.size and 0 < st["max_distinct"] <= 2048
+    assert st["blocks"] >= x.size // 256
+    ia = R.make_row_ptr(x.size, n)
+    for c, a in enumerate(_planned(plan, w, u)):
+        yr, _ = O.apply(ia, ja, w[c].cpu().numpy(), u)
+        np.testing.assert_array_equal(a, yr)
+    # operator subsets select the same rows
+    y2 = _planned(plan, w[1:], u)
+    np.testing.assert_array_equal(y2[0], O.apply(ia, ja, w[1].cpu().numpy(), u)[0])
+
+
+def test_planned_apply_ragged_with_boundary_identity():
+    import torch
+    from rbf_amd import workloads
+    W = workloads.build_disk(9000, 300, seed=34)
+    ia, ja, x, y = W["ia"], W["ja"], W["x"], W["y"]
+    b = R.Batch(workloads.DISK_CLASSES)
+    w = [torch.zeros(int(ia[-1]), dtype=torch.float64, device="cuda:0") for _ in range(3)]
+    b.bind(ia, _dev(x), _dev(y), _dev(ia), _dev(ja), w)
+    b.launch()
+    torch.cuda.synchronize()
+    bnd = np.nonzero(W["flag"])[0].astype(np.int32)
+    u = np.exp(x) * np.sin(y) + 0.5
+    plan = R.ApplyPlan(ja, ia=ia, bnd=bnd)
+    for c, a in enumerate(_planned(plan, w, u)):
+        yr, _ = O.apply(ia, ja, w[c].cpu().numpy(), u, bnd)
+        np.testing.assert_array_equal(a, yr)
+    # four operators (the fourth a copy of the first) through the widest kernel
+    w4 = w + [w[0].clone()]
+    y4 = _planned(plan, w4, u)
+    np.testing.assert_array_equal(y4[3], y4[0])
+

@@ -53,6 +53,28 @@ def test_plan_argument_checks():
     assert e.value.status == R.ERR_UNSUPPORTED
 
 
+def test_apply_argument_checks():
+    L = R.lib()
+    tab = (ctypes.c_void_p * 1)(None)
+    assert L.rbfa_apply(10, 35, None, None, 5, tab, None, tab, 0, None, None) == R.ERR_ARG
+    assert L.rbfa_apply(10, 35, None, None, 1, tab, None, tab, 0, None, None) == R.ERR_ARG
+    assert L.rbfa_apply(10, 35, None, None, 1, tab, None, tab, 3, None, None) == R.ERR_ARG
+    assert L.rbfa_apply(0, 35, None, None, 1, tab, None, tab, 0, None, None) == R.OK
+
+
+def test_apply_plan_validates_structure_before_touching_a_device():
+    import numpy as np
+    with pytest.raises(R.RbfError) as e:
+        R.ApplyPlan(np.array([0, 1, 5, 2], np.int32), k=2)      # column 5 of 2 columns
+    assert e.value.status == R.ERR_ARG
+    with pytest.raises(R.RbfError) as e:
+        R.ApplyPlan(np.array([0, 1], np.int32), ia=np.array([0, 2, 1], np.int32))
+    assert e.value.status == R.ERR_ARG                           # decreasing row pointers
+    with pytest.raises(R.RbfError) as e:
+        R.ApplyPlan(np.zeros(3000, np.int32), ia=np.array([0, 3000], np.int32), ncols=1)
+    assert e.value.status == R.ERR_ARG                           # a row wider than 2048
+
+
 def test_assemble_rejects_null_plan():
     assert R.lib().rbfa_assemble(None, 10, None, None, None, None, None, None, None, None, None,
                                  None, None) == R.ERR_ARG

@@ -269,3 +269,33 @@ def test_knn_brute_matches_reference_nanoflann():
 def test_knn_brute_matches_reference_ckdtree_3d():
     g = np.load(os.path.join(GOLD, "ja_cube_1000_k80.npz"))
     np.testing.assert_array_equal(O.knn_brute(80, g["x"], g["y"], g["z"]), g["ja"])
+
+
+def test_apply_against_scipy_csr_and_closed_form():
+    """The oracle's mv_op (main.cpp:59-66) against scipy's CSR product on a
+    ragged random matrix, the boundary rows against the identity, and, on
+    assembled weights, the Laplacian of x^2 + y^2 (= 4) as a closed form."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(5)
+    nr = 300
+    lens = rng.integers(1, 40, nr)
+    ia = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    ja = rng.integers(0, nr, ia[-1]).astype(np.int32)
+    w = rng.standard_normal(ia[-1])
+    u = rng.standard_normal(nr)
+    A = sp.csr_matrix((w, ja, ia), shape=(nr, nr))
+    y, scale = O.apply(ia, ja, w, u)
+    np.testing.assert_allclose(y, A @ u, rtol=0, atol=1e-13 * scale.max())
+    absA = sp.csr_matrix((np.abs(w), ja, ia), shape=(nr, nr))
+    np.testing.assert_allclose(scale, absA @ np.abs(u), rtol=1e-14)
+    bnd = np.array([3, 77, 299], np.int32)
+    y2, _ = O.apply(ia, ja, w, u, bnd)
+    np.testing.assert_array_equal(y2[bnd], u[bnd])
+    keep = np.setdiff1d(np.arange(nr), bnd)
+    np.testing.assert_array_equal(y2[keep], y[keep])
+    x, yy = np.meshgrid((np.arange(24) + 0.5) / 24, (np.arange(24) + 0.5) / 24)
+    x, yy = x.ravel(), yy.ravel()
+    jn = O.knn_brute(21, x, yy)
+    wl, info = O.assemble(2, 21, 2, 3, x, yy, jn)
+    lap, _ = O.apply(np.arange(0, jn.size + 1, 21), jn, wl[0], x * x + yy * yy)
+    np.testing.assert_allclose(lap, 4.0, atol=1e-8)
