@@ -230,6 +230,131 @@ def run_apply(args, R, workloads, torch, dev):
     return W, res, cpu
 
 
+def run_fixed(args, name, R, workloads, torch, dev):
+    W = workloads.build(name, seed=args.seed)
+    dim, n, P, Q, nst = W["dim"], W["n"], W["P"], W["Q"], W["nstencils"]
+    log(f"[{name}] host prep: {W['host_s']}")
+    pts = [W["x"], W["y"]] + ([W["z"]] if dim == 3 else [])
+    dpts = [torch.from_numpy(a).to(dev) for a in pts]
+    ja = torch.from_numpy(W["ja"]).to(dev)
+    nnz = nst * n
+    info = torch.empty(nst, dtype=torch.int32, device=dev)
+    status = torch.empty(nst, dtype=torch.uint8, device=dev)
+    forms = ["fp64", "mixed"] if args.form == "both" else [args.form]
+    res = {}
+    for form in forms:
+        try:
+            plan = R.Plan(dim, n, P, Q, "diff", precision=form, max_refine=3)
+        except R.RbfError as e:
+            log(f"[{form}] {e}")
+            continue
+        w = [torch.empty(nnz, dtype=torch.float64, device=dev) for _ in range(plan.nops)]
+        z = dpts[2] if dim == 3 else None
+
+        def step():
+            plan.assemble(nst, dpts[0], dpts[1], ja, w, z=z, info=info, status=status)
+
+        tune_ms = None
+        if args.variant is not None:
+            plan.variant = args.variant
+        elif not args.no_tune and plan.num_variants > 1:
+            tune_ms = plan.autotune(nst, dpts[0], dpts[1], ja, w, z=z, info=info, status=status,
+                                    reps=2)
+            log(f"[{form}] autotune: " + ", ".join(
+                f"{plan.variant_name(v)}={t:.3f}ms" for v, t in enumerate(tune_ms)))
+        log(f"[{form}] variant {plan.variant_name()}; warmup {args.warmup}, "
+            f"timing {args.steps} steps")
+        ms, wall = hip_timed(step, args.steps, args.warmup)
+        st = status.cpu().numpy()
+        res[form] = {"ms": ms, "wall_s": wall, "singular": int((info != 0).sum().item()),
+                     "fallback": int(((st & R.STAT_FALLBACK) != 0).sum()) if form == "mixed" else 0,
+                     "refine_steps_hist": np.bincount(st & 3, minlength=4).tolist()
+                     if form == "mixed" else None,
+                     "variant": plan.variant_name(),
+                     "autotune_ms": None if tune_ms is None else
+                     {plan.variant_name(v): round(t, 4) for v, t in enumerate(tune_ms)},
+                     "nt": plan.nt, "r": plan.nops, "w": [t.cpu().numpy() for t in w]}
+        log(f"[{form}] {ms:.3f} ms/step, {nst / (ms * 1e-3) / 1e6:.2f} Mstencils/s, "
+            f"singular {res[form]['singular']}, fallback {res[form]['fallback']}")
+        del w
+        torch.cuda.empty_cache()
+    return W, pts, res, n, dim
+
+
+def oracle_fixed(W, res, n, dim, sample):
+    from oracle import oracle as O
+    cores = len(os.sched_getaffinity(0))
+    nst = W["nstencils"]
+    cnt = min(sample, nst)
+    s0 = (nst - cnt) // 2
+    log(f"CPU oracle: {cnt} stencils on {cores} threads")
+    t0 = time.perf_counter()
+    wref, _ = O.assemble(dim, n, W["P"], W["Q"], W["x"], W["y"], W["ja"], z=W["z"], s0=s0,
+                         count=cnt, nthreads=cores)
+    cpu_s = time.perf_counter() - t0
+    cpu = {"value": cnt / cpu_s, "unit": "stencils/s", "cores": cores, "kind": "port",
+           "sample": f"{cnt} contiguous stencils (from {s0}) of the same workload, FP64 oracle "
+                     f"(gather, fill, partial-pivot LU, solve, scatter), {cpu_s:.2f} s"}
+    errs = {}
+    for form, d in res.items():
+        wg = [a[s0 * n:(s0 + cnt) * n] for a in d["w"]]
+        e = rel_errors(wg, wref, n)
+        errs[form] = {"weight_rel_err_max": float(e.max()),
+                      "weight_rel_err_median": float(np.median(e)), "oracle_sample": cnt}
+        log(f"[{form}] weight rel err max {e.max():.3e} median {np.median(e):.3e}")
+    return cpu, errs
+
+
+def run_disk(args, R, workloads, torch, dev):
+    W = workloads.build_disk(seed=args.seed + 3)
+    log(f"[disk] host prep: {W['host_s']}")
+    nst, ia, jah = W["nstencils"], W["ia"], W["ja"]
+    nnz = int(ia[-1])
+    x, y = (torch.from_numpy(a).to(dev) for a in (W["x"], W["y"]))
+    iad, jad = torch.from_numpy(ia).to(dev), torch.from_numpy(jah).to(dev)
+    w = [torch.empty(nnz, dtype=torch.float64, device=dev) for _ in range(3)]
+    info = torch.empty(nst, dtype=torch.int32, device=dev)
+    b = R.Batch(workloads.DISK_CLASSES)
+    b.bind(ia, x, y, iad, jad, w, info=info)
+    counts = {f"n{c[0]}": b.class_count(i) for i, c in enumerate(b.classes)}
+    tune = b.autotune(reps=2) if not args.no_tune else None
+    if tune:
+        log(f"[disk] autotune: per-class launches {tune[0]:.3f} ms, persistent {tune[1]:.3f} ms; "
+            f"mode {b.mode}; variants "
+            + ", ".join(f"n{c[0]}: {b.class_variant(i)}" for i, c in enumerate(b.classes)))
+    ms, wall = hip_timed(b.launch, args.steps, args.warmup)
+    log(f"[disk] {ms:.3f} ms/step, {nst / (ms * 1e-3) / 1e6:.2f} Mstencils/s")
+    res = {"fp64": {"ms": ms, "wall_s": wall, "singular": int((info != 0).sum().item()),
+                    "fallback": 0, "refine_steps_hist": None,
+                    "variant": "mode %d (%s)" % (b.mode, "persistent" if b.mode else "per-class"),
+                    "autotune_ms": None if tune is None else
+                    {"per_class_launches": round(tune[0], 4), "persistent": round(tune[1], 4)},
+                    "class_counts": counts, "w": [t.cpu().numpy() for t in w]}}
+    return W, res
+
+
+def oracle_disk(W, res):
+    from oracle import oracle as O
+    from rbf_amd import workloads
+    cores = len(os.sched_getaffinity(0))
+    ia, ja = W["ia"], W["ja"]
+    nnz = int(ia[-1])
+    wref = [np.zeros(nnz) for _ in range(3)]
+    iref = np.zeros(W["nstencils"], np.int32)
+    t0 = time.perf_counter()
+    for (n, P, Q) in workloads.DISK_CLASSES:
+        rows = np.nonzero(W["sizes"] == n)[0].astype(np.int32)
+        O.assemble_rows(2, n, P, Q, rows, ia, W["x"], W["y"], ja, wref, iref, nthreads=cores)
+    cpu_s = time.perf_counter() - t0
+    cpu = {"value": W["nstencils"] / cpu_s, "unit": "stencils/s", "cores": cores, "kind": "port",
+           "sample": f"all {W['nstencils']} stencils of the workload, FP64 oracle, {cpu_s:.2f} s"}
+    e = rel_errors(res["fp64"]["w"], wref, ia)
+    log(f"[disk] weight rel err max {e.max():.3e} median {np.median(e):.3e}")
+    return cpu, {"fp64": {"weight_rel_err_max": float(e.max()),
+                          "weight_rel_err_median": float(np.median(e)),
+                          "oracle_sample": W["nstencils"]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)

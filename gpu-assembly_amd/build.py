@@ -21,9 +21,12 @@ ARCH = "gfx950"
 # Packed FP32 VALU forms stay off (the compiler would otherwise fuse pairs
 # of float ops into v_pk_* instructions); contraction stays off so sums of
 # products round exactly as written, FMAs being explicit fma() calls.
+# MFMA accumulators stay in ordinary VGPRs (by default LLVM moves them to
+# AGPRs, and a kernel's register budget is the sum of both files).
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wno-unused-command-line-argument", f"-I{INC}",
 ]
 CXX_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-fopenmp", "-ffp-contract=off", "-march=x86-64-v2",

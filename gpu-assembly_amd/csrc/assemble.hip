@@ -145,7 +145,8 @@ constexpr Shape diff64() {
 }
 template <int DIM, int N, int P, int Q>
 struct GroupVariants {
-    static constexpr Variant list[1] = {
+    static constexpr Variant list[2] = {
+        {"blocked lu fp64 mfma", &launch_tile_diff<DIM, N, P, Q>},
         {"workgroup nw=2 bc=lds pv=dpp", &launch_group_diff<DIM, N, P, Q>}};
 };
 template <int DIM, int N, int P, int Q>
@@ -163,7 +164,7 @@ template <int DIM, int N, int P, int Q>
 constexpr Shape diffgroup() {
     constexpr int M = npoly_dim(DIM, P);
     return {DIM, N, P, Q, RBFA_OPS_DIFF, 0, RBFA_PREC_FP64, N + M, DIM + 1,
-            GroupVariants<DIM, N, P, Q>::list, 1};
+            GroupVariants<DIM, N, P, Q>::list, 2};
 }
 template <int N, int P, int Q, int R>
 constexpr Shape interp64() {

@@ -41,6 +41,11 @@ template <int DIM, int N, int P, int Q>
 hipError_t launch_group_diff(const LaunchArgs& a);
 template <int DIM, int N, int P, int Q>
 hipError_t launch_group_mixed(const LaunchArgs& a);
+// Systems of 64 unknowns and more, blocked LU with FP64 MFMA trailing
+// updates, one workgroup per stencil, one wave per 16-column tile
+// (assemble_tile.hip).
+template <int DIM, int N, int P, int Q>
+hipError_t launch_tile_diff(const LaunchArgs& a);
 
 }  // namespace rbf_amd_internal
 

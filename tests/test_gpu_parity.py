@@ -148,18 +148,26 @@ def test_singular_stencil_reports_info():
 
 @pytest.mark.parametrize("dim,m,n,P,Q", [(2, 32, 56, 4, 7), (3, 10, 80, 3, 3)])
 def test_large_systems_match_oracle(dim, m, n, P, Q):
-    """Systems larger than a wavefront (71x71 in 2-D, 100x100 in 3-D), one
-    two-wave workgroup per stencil."""
+    """Systems larger than a wavefront (71x71 in 2-D, 100x100 in 3-D), every
+    engine: the two-wave row-per-lane workgroup and the blocked LU with
+    FP64 MFMA trailing updates. Both apply the oracle's operations in the
+    oracle's order, so the weights are bit-identical to it."""
     plan = R.Plan(dim, n, P, Q, "diff")
     pts = R.jittered_grid(dim, m, 0.25, seed=13)
     ja = R.knn_stencils(n, *pts)
     z = pts[2] if dim == 3 else None
-    w, info, status = _run(plan, pts[0], pts[1], ja, z=z)
     wref, iref = O.assemble(dim, n, P, Q, pts[0], pts[1], ja, z=z)
-    assert (info == 0).all() and (iref == 0).all()
-    e = _relerr(w, wref, n)
-    print(f"dim={dim} n={n} nt={plan.nt}: max rel err {e.max():.3e}")
-    assert e.max() <= FP64_TOL
+    assert (iref == 0).all()
+    assert plan.num_variants >= 2
+    for v in range(plan.num_variants):
+        plan.variant = v
+        w, info, status = _run(plan, pts[0], pts[1], ja, z=z)
+        assert (info == 0).all()
+        e = _relerr(w, wref, n)
+        print(f"dim={dim} n={n} nt={plan.nt} {plan.variant_name()}: max rel err {e.max():.3e}")
+        assert e.max() <= FP64_TOL
+        for a, b in zip(w, wref):
+            np.testing.assert_array_equal(a, b)
 
 
 def _disk_case(n_int=12000, n_bnd=300, seed=21):
