@@ -66,7 +66,9 @@ struct TileAssembly {
 
     // LDS (doubles): [fill: points, monomials | panel row block [MP][S]],
     // multiplier panels [2][MP][S] (reused as the back-substitution block
-    // buffers), U staging [W][16][16], pivot reciprocals [NT], per-wave
+    // buffers), U staging [W][16][16], the multipliers of each panel's
+    // pivot rows in step order [2][16][16], the pivot lane's two rows [2][16],
+    // pivot reciprocals [NT], per-wave
     // maxima [W], then ints: pivot row of each step [NT], step of each row
     // [MP], info.
     static constexpr int FILL = DIM * N + M * N;
@@ -77,7 +79,9 @@ struct TileAssembly {
     static constexpr int off_x = 0;
     static constexpr int off_l = R0;
     static constexpr int off_u = off_l + 2 * XB;
-    static constexpr int off_rcp = off_u + W * 256;
+    static constexpr int off_l11 = off_u + W * 256;
+    static constexpr int off_prow = off_l11 + 2 * 256;
+    static constexpr int off_rcp = off_prow + 32;
     static constexpr int off_wmax = off_rcp + (NT + 1) / 2 * 2;
     static constexpr int off_int = off_wmax + (W + 1) / 2 * 2;
     static constexpr int int_perm = 0, int_step = NT, int_info = NT + MP;
@@ -163,53 +167,80 @@ struct TileAssembly {
     }
 
     // ------------------------------------------------------------------ panel
-    // Update panel columns J.. of both row sets with the pivot row (lane wl
-    // of set ws).
+    // Pivot of step k from its column (a0: rows 0..63, a1: rows 64..): the
+    // largest |a| among active rows (NaN never wins), the lowest row on
+    // ties, else the first active row with info = k + 1; info = k + 1 also
+    // for a zero pivot. Each lane first keeps the better of its two rows
+    // (row lane before row 64 + lane on ties), then one wave maximum on the
+    // high word of |a| (low word only when high words tie). Returns the
+    // global row and 1 / pivot.
+    __device__ __forceinline__ static int pivot_of(double a0, double a1, bool act0, bool act1,
+                                                   int k, int& info, double& rcp) {
+        const bool c0 = act0 && !__builtin_isnan(a0);
+        const bool c1 = act1 && !__builtin_isnan(a1);
+        const unsigned long long k0 = static_cast<unsigned long long>(__double_as_longlong(fabs(a0)));
+        const unsigned long long k1 = static_cast<unsigned long long>(__double_as_longlong(fabs(a1)));
+        const bool use1 = c1 && (!c0 || k1 > k0);
+        const bool cb = c0 || c1;
+        const unsigned long long kb = use1 ? k1 : k0;
+        const unsigned hi = cb ? static_cast<unsigned>(kb >> 32) : 0u;
+        const unsigned mh = wave_umax(hi);
+        bool top = cb && hi == mh;
+        const unsigned long long any = __ballot(top);
+        if (__popcll(any) > 1) {
+            const unsigned lo = top ? static_cast<unsigned>(kb) : 0u;
+            const unsigned ml = wave_umax(lo);
+            top = top && lo == ml;
+        }
+        const unsigned long long h0 = __ballot(top && !use1);
+        const unsigned long long h1 = __ballot(top && use1);
+        int win;
+        if (h0) {
+            win = __ffsll(h0) - 1;
+        } else if (h1) {
+            win = 64 + __ffsll(h1) - 1;
+        } else {  // no candidate that is a number: first row still active
+            const unsigned long long b0 = __ballot(act0);
+            const unsigned long long b1 = __ballot(act1);
+            win = b0 ? __ffsll(b0) - 1 : (b1 ? 64 + __ffsll(b1) - 1 : 0);
+            if (!info) info = k + 1;
+        }
+        win = __builtin_amdgcn_readfirstlane(win);
+        const double piv = bcast(win >= 64 ? a1 : a0, win & 63);
+        if (piv == 0.0 && !info) info = k + 1;
+        rcp = 1.0 / piv;
+        return win;
+    }
+
+    // Update panel columns J.. of both row sets with the pivot row's values
+    // u[J..] (read back from LDS at one address: a broadcast).
     template <int J>
-    __device__ __forceinline__ static void panel_update(double (&p0)[16], double (&p1)[16],
-                                                        double l0, double l1, bool ws, int wl) {
+    __device__ __forceinline__ static void panel_update_lds(double (&p0)[16], double (&p1)[16],
+                                                            double l0, double l1, const double* pr) {
         if constexpr (J < 16) {
-            const double u = bcast(ws ? p1[J] : p0[J], wl);
+            const double u = pr[J];
             p0[J] = fma(-l0, u, p0[J]);
             p1[J] = fma(-l1, u, p1[J]);
-            if constexpr ((J & 3) == 3) __builtin_amdgcn_sched_barrier(0);
-            panel_update<J + 1>(p0, p1, l0, l1, ws, wl);
+            panel_update_lds<J + 1>(p0, p1, l0, l1, pr);
         }
     }
 
-    // Elimination step C (compile time) of the panel starting at step k0.
+    // Elimination step C (compile time) of the panel starting at step k0,
+    // whose pivot (win, rcp) is known. Lookahead: column C+1 is updated
+    // first and the pivot of step C+1 found before the remaining columns
+    // are updated, so the pivot search and the division overlap them.
     template <int C, int KQ, bool STORE_L>
     __device__ __forceinline__ static void panel_step(double (&p0)[16], double (&p1)[16],
                                                       bool& act0, bool& act1, int k0, double* lb,
-                                                      double* lds, int& info, int lane) {
+                                                      double* lds, int& info, int lane, int win,
+                                                      double rcp) {
         if constexpr (C < KQ) {
             int* ints = reinterpret_cast<int*>(lds + off_int);
             const int k = k0 + C;
             const double a0 = p0[C];
             const double a1 = p1[C];
-            int q0 = pivot_lane<1>(a0, act0);
-            int q1 = pivot_lane<1>(a1, act1);
-            int win;
-            if (q0 >= 0 && q1 >= 0) {
-                q0 = __builtin_amdgcn_readfirstlane(q0);
-                q1 = __builtin_amdgcn_readfirstlane(q1);
-                win = fabs(bcast(a0, q0)) >= fabs(bcast(a1, q1)) ? q0 : 64 + q1;
-            } else if (q0 >= 0) {
-                win = q0;
-            } else if (q1 >= 0) {
-                win = 64 + q1;
-            } else {  // no candidate that is a number: first row still active
-                const unsigned long long b0 = __ballot(act0);
-                const unsigned long long b1 = __ballot(act1);
-                win = b0 ? __ffsll(b0) - 1 : (b1 ? 64 + __ffsll(b1) - 1 : 0);
-                if (!info) info = k + 1;
-            }
-            win = __builtin_amdgcn_readfirstlane(win);
             const int wl = win & 63;
             const bool ws = win >= 64;
-            const double piv = bcast(ws ? a1 : a0, wl);
-            if (piv == 0.0 && !info) info = k + 1;
-            const double rcp = 1.0 / piv;
             const bool ip0 = !ws && lane == wl;
             const bool ip1 = ws && lane == wl;
             const double l0 = (act0 && !ip0) ? a0 * rcp : 0.0;
@@ -227,8 +258,36 @@ struct TileAssembly {
                 ints[int_step + win] = k;
                 lds[off_rcp + k] = rcp;
             }
-            panel_update<C + 1>(p0, p1, l0, l1, ws, wl);
-            panel_step<C + 1, KQ, STORE_L>(p0, p1, act0, act1, k0, lb, lds, info, lane);
+            // the pivot lane's two rows' columns C+2.. to LDS, the pivot row
+            // read back below (the slot is chosen by address, not by a select)
+            double* pr = lds + off_prow + (ws ? 16 : 0);
+            if constexpr (C + 2 < 16) {
+                if (lane == wl) {
+                    double* q0 = lds + off_prow;
+#pragma unroll
+                    for (int J = C + 2; J < 16; ++J) q0[J] = p0[J];
+#pragma unroll
+                    for (int J = C + 2; J < 16; ++J) q0[16 + J] = p1[J];
+                }
+            }
+            if constexpr (C + 1 < 16) {
+                const double u = bcast(ws ? p1[C + 1] : p0[C + 1], wl);
+                p0[C + 1] = fma(-l0, u, p0[C + 1]);
+                p1[C + 1] = fma(-l1, u, p1[C + 1]);
+            }
+            if constexpr (C + 1 < KQ) {
+                double nrcp;
+                const int nwin = pivot_of(p0[C + 1], p1[C + 1], act0, act1, k + 1, info, nrcp);
+                wave_lds_sync();
+                panel_update_lds<C + 2>(p0, p1, l0, l1, pr);
+                wave_lds_sync();  // reads done before the next step's row is written
+                panel_step<C + 1, KQ, STORE_L>(p0, p1, act0, act1, k0, lb, lds, info, lane, nwin,
+                                               nrcp);
+            } else {
+                wave_lds_sync();
+                panel_update_lds<C + 2>(p0, p1, l0, l1, pr);
+                wave_lds_sync();
+            }
         }
     }
 
@@ -284,10 +343,22 @@ struct TileAssembly {
         bool act0 = lane < NT && ints[int_step + lane] == NONE;
         bool act1 = lane + 64 < NT && ints[int_step + (lane + 64 < MP ? lane + 64 : 0)] == NONE;
         int info = 0;
+        double rcp0;
+        const int win0 = pivot_of(p0[0], p1[0], act0, act1, 16 * q, info, rcp0);
         panel_step<0, KQ, STORE_L>(p0, p1, act0, act1, 16 * q, lds + off_l + (q & 1) * XB, lds,
-                                   info, lane);
+                                   info, lane, win0, rcp0);
         wave_lds_sync();
         store_rows(p0, p1, X, lane);
+        if constexpr (STORE_L) {
+            // the pivot rows' multipliers, row kk = pivot of step 16q + kk
+            const double* lb = lds + off_l + (q & 1) * XB;
+            double* l11 = lds + off_l11 + (q & 1) * 256;
+            if (lane < 16) {
+                const int prow = ints[int_perm + 16 * q + lane];
+#pragma unroll
+                for (int t = 0; t < 16; ++t) l11[lane * 16 + t] = lb[prow * S + t];
+            }
+        }
         wave_lds_sync();
         if constexpr (STORE_L) rows_to_tile(tile, X, lane);
         if (lane == 0 && info && ints[int_info] == 0) ints[int_info] = info;
@@ -313,10 +384,10 @@ struct TileAssembly {
 #pragma unroll
         for (int kk = 0; kk < 16; ++kk) u[kk] = ub[kk * 16 + col];
         wave_lds_sync();  // reads done before this buffer is reused
+        const double* l11 = lds + off_l11 + (q & 1) * 256;
 #pragma unroll
         for (int kk = 1; kk < 16; ++kk) {
-            const int prow = __builtin_amdgcn_readfirstlane(ints[int_perm + 16 * q + kk]);
-            const double* lr = lb + prow * S;
+            const double* lr = l11 + kk * 16;
 #pragma unroll
             for (int s = 0; s < kk; ++s) u[kk] = fma(lr[s], u[s], u[kk]);
             // computed here: left alone, the (pure) chains sink to their use
@@ -326,53 +397,48 @@ struct TileAssembly {
     }
 
     // Trailing update of this wave's column tiles right of panel q, one
-    // slot at a time (U block, then four chained MFMAs per row tile).
+    // slot at a time (U block, then four chained MFMAs per row tile). The
+    // owner of panel q + 1 (when it is not the last) factors it as soon as
+    // that tile is updated, before its other tiles: the next panel's
+    // factorisation is the critical path.
     __device__ __forceinline__ static void trailing(dbl4 (&tile)[LC][RT], int q, double* lds,
                                                     int w, int lane) {
         const double* lb = lds + off_l + (q & 1) * XB;
         double* ub = lds + off_u + w * 256;
         const int r = lane >> 4, col = lane & 15;
+        const int jn = (q + 1) / W;  // slot of tile q + 1 in its owner
 #pragma unroll
-        for (int j = 0; j < LC; ++j) {
-            const int C = w + W * j;
-            if (C > q && C < CT) {
-                double bop[4];
-                {
-                    double u[16];
-                    u_block(tile[j], q, lds, ub, lb, lane, u);
+        for (int pass = 0; pass < 2; ++pass) {
 #pragma unroll
-                    for (int c = 0; c < 4; ++c)
-                        bop[c] = r == 0 ? u[4 * c]
-                                        : (r == 1 ? u[4 * c + 1] : (r == 2 ? u[4 * c + 2] : u[4 * c + 3]));
-                }
+            for (int j = 0; j < LC; ++j) {
+                const int C = w + W * j;
+                // pass 0: the next panel's tile (its owner only); pass 1: the rest
+                const bool next = C == q + 1;
+                if (C > q && C < CT && next == (pass == 0)) {
+                    double bop[4];
+                    {
+                        double u[16];
+                        u_block(tile[j], q, lds, ub, lb, lane, u);
 #pragma unroll
-                for (int T = 0; T < RT; ++T) {
-                    double a[4];
+                        for (int c = 0; c < 4; ++c)
+                            bop[c] = r == 0 ? u[4 * c]
+                                            : (r == 1 ? u[4 * c + 1]
+                                                      : (r == 2 ? u[4 * c + 2] : u[4 * c + 3]));
+                    }
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) a[c] = lb[(16 * T + col) * S + 4 * c + r];
+                    for (int T = 0; T < RT; ++T) {
+                        double a[4];
 #pragma unroll
-                    for (int c = 0; c < 4; ++c)
-                        tile[j][T] =
-                            __builtin_amdgcn_mfma_f64_16x16x4f64(a[c], bop[c], tile[j][T], 0, 0, 0);
+                        for (int c = 0; c < 4; ++c) a[c] = lb[(16 * T + col) * S + 4 * c + r];
+#pragma unroll
+                        for (int c = 0; c < 4; ++c)
+                            tile[j][T] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[c], bop[c],
+                                                                               tile[j][T], 0, 0, 0);
+                    }
+                    if (pass == 0 && q + 1 < RT - 1 && j == jn)
+                        factor<16, true>(tile[j], q + 1, lds, lane);
                 }
             }
-        }
-    }
-
-    // Panels J*W .. J*W + W - 1 (slot J of their owners), all but the last
-    // panel: factorisation by the owner, one barrier, trailing updates.
-    template <int J>
-    __device__ __forceinline__ static void panel_slots(dbl4 (&tile)[LC][RT], double* lds, int w,
-                                                       int lane) {
-        if constexpr (J < LC) {
-            for (int o = 0; o < W; ++o) {
-                const int q = J * W + o;
-                if (q >= RT - 1) break;
-                if (w == o) factor<16, true>(tile[J], q, lds, lane);
-                __syncthreads();
-                trailing(tile, q, lds, w, lane);
-            }
-            panel_slots<J + 1>(tile, lds, w, lane);
         }
     }
 
@@ -449,7 +515,11 @@ struct TileAssembly {
             }
         }
         __syncthreads();  // points and monomials no longer needed
-        panel_slots<0>(tile, lds, w, lane);
+        if (w == 0) factor<16, true>(tile[0], 0, lds, lane);
+        for (int q = 0; q < RT - 1; ++q) {
+            __syncthreads();  // panel q published
+            trailing(tile, q, lds, w, lane);
+        }
         constexpr int BW = (RT - 1) % W;  // owner of the last panel and of the right-hand sides
         constexpr int JB = (RT - 1) / W;
         constexpr int K0 = 16 * (RT - 1);
