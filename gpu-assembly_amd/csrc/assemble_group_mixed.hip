@@ -6,6 +6,7 @@
 
 #include "assemble_common.h"
 #include "rbf_group.h"
+#include "rbf_tile.h"
 
 namespace rbf_amd_internal {
 
@@ -29,9 +30,9 @@ k_assemble_group_mixed(int nstencils, const double* __restrict__ x, const double
 }
 
 // The FP64 solve of the stencils the mixed kernel could not converge,
-// persistent over the fallback list.
+// persistent over the fallback list (the blocked-LU engine, four waves).
 template <class G>
-__global__ void __launch_bounds__(128)
+__global__ void __launch_bounds__(G::threads)
 k_group_fallback(const double* __restrict__ x, const double* __restrict__ y,
                  const double* __restrict__ z, const int* __restrict__ ja,
                  detail::ValueOut<double, G::nrhs> out, int* __restrict__ info,
@@ -49,7 +50,7 @@ template <int DIM, int N, int P, int Q>
 hipError_t launch_group_mixed(const LaunchArgs& a) {
     constexpr int NW = 2;
     using E = detail::GroupMixed<DIM, N, P, Q, NW>;
-    using G = detail::GroupAssembly<DIM, N, P, Q, DIM + 1, false, NW>;
+    using G = detail::TileAssembly<DIM, N, P, Q>;
     detail::ValueOut<double, E::nrhs> out;
     for (int c = 0; c < E::nrhs; ++c) out.p[c] = a.w[c];
     hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(64), 0, a.stream, a.ws);
@@ -60,7 +61,7 @@ hipError_t launch_group_mixed(const LaunchArgs& a) {
                        a.max_refine, a.ws);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_group_fallback<G>), dim3(1024), dim3(64 * NW), 0, a.stream, a.x, a.y, a.z,
+    hipLaunchKernelGGL((k_group_fallback<G>), dim3(1024), dim3(G::threads), 0, a.stream, a.x, a.y, a.z,
                        a.ja, out, a.info, a.ws);
     return hipGetLastError();
 }

@@ -3,8 +3,9 @@
 //   mode 0: one launch per size class over that class's stencil list, or
 //   mode 1: one persistent launch whose workgroups draw stencils from
 //           size-bucketed work queues (atomic tickets): the largest class
-//           first, a whole two-wave workgroup per stencil, then each wave
-//           on its own through the smaller classes.
+//           first, a whole four-wave workgroup per stencil (blocked LU with
+//           FP64 MFMA trailing updates), then each wave on its own through
+//           the smaller classes.
 // Either pipeline is captured once into a hipGraph at bind time and
 // replayed by rbfa_batch_launch; rbfa_batch_autotune picks each class's
 // kernel variant and the faster pipeline.
@@ -24,6 +25,7 @@
 #include "rbf_assembly.h"
 #include "rbf_group.h"
 #include "rbf_operators.h"
+#include "rbf_tile.h"
 
 namespace {
 
@@ -53,8 +55,8 @@ k_list_wave(int count, const int* __restrict__ order, const int* __restrict__ ia
     if (status && detail::lane_id() == 0) status[s] = 0;
 }
 
-template <class G>
-__global__ void __launch_bounds__(128)
+template <class G, int THREADS>
+__global__ void __launch_bounds__(THREADS)
 k_list_group(int count, const int* __restrict__ order, const int* __restrict__ ia,
              const double* __restrict__ x, const double* __restrict__ y,
              const double* __restrict__ z, const int* __restrict__ ja, Out3 out,
@@ -82,9 +84,10 @@ __global__ void k_zero(int* p, int n) {
 
 template <class G, class E2, class E1, class E0>
 struct Persistent {
+    static constexpr int waves = G::threads / 64;
     static constexpr int wave_lds =
         std::max(E2::lds_doubles, std::max(E1::lds_doubles, E0::lds_doubles));
-    static constexpr int lds_doubles = std::max(G::lds_doubles, 2 * wave_lds);
+    static constexpr int lds_doubles = std::max(G::lds_doubles, waves * wave_lds);
 
     template <class E>
     __device__ static void drain(const Queues& q, int c, const int* __restrict__ ia,
@@ -110,9 +113,11 @@ using EP13 = detail::WaveAssembly<2, 13, 2, 3, 3, false, detail::LuTune<2, 1>>;
 using EP21 = detail::WaveAssembly<2, 21, 2, 3, 3, false, detail::LuTune<2, 1>>;
 using EP35 = detail::WaveAssembly<2, 35, 3, 5, 3, false, detail::LuTune<2, 1>>;
 using GP56 = detail::GroupAssembly<2, 56, 4, 7, 3, false, 2>;
-using PK = Persistent<GP56, EP35, EP21, EP13>;
+using TP56 = detail::TileAssembly<2, 56, 4, 7>;
+using PK = Persistent<TP56, EP35, EP21, EP13>;
+constexpr int kPersistentThreads = TP56::threads;
 
-__global__ void __launch_bounds__(128)
+__global__ void __launch_bounds__(kPersistentThreads)
 k_persistent(Queues q, const int* __restrict__ ia, const double* __restrict__ x,
              const double* __restrict__ y, const int* __restrict__ ja, Out3 out,
              int* __restrict__ info, uint8_t* __restrict__ status) {
@@ -126,8 +131,9 @@ k_persistent(Queues q, const int* __restrict__ ia, const double* __restrict__ x,
         __syncthreads();
         if (t >= q.count[3]) break;
         const int s = q.order[q.start[3] + t];
-        GP56::run(s, static_cast<size_t>(ia[s]), x, y, nullptr, ja, out, lds, info);
+        TP56::run(s, static_cast<size_t>(ia[s]), x, y, nullptr, ja, out, lds, info);
         if (status && threadIdx.x == 0) status[s] = 0;
+        __syncthreads();  // LDS reuse by the next stencil
     }
     // then each wave on its own, larger classes first
     double* wl = lds + (threadIdx.x >> 6) * PK::wave_lds;
@@ -159,10 +165,10 @@ hipError_t launch_list_wave(const ListArgs& a) {
                        a.count, a.order, a.ia, a.x, a.y, a.z, a.ja, a.out, a.info, a.status);
     return hipGetLastError();
 }
-template <class G>
+template <class G, int THREADS>
 hipError_t launch_list_group(const ListArgs& a) {
     if (a.count == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_list_group<G>), dim3(xcd_grid(a.count)), dim3(128), 0, a.stream,
+    hipLaunchKernelGGL((k_list_group<G, THREADS>), dim3(xcd_grid(a.count)), dim3(THREADS), 0, a.stream,
                        a.count, a.order, a.ia, a.x, a.y, a.z, a.ja, a.out, a.info, a.status);
     return hipGetLastError();
 }
@@ -172,7 +178,7 @@ struct ListVariant {
     ListFn fn;
 };
 struct BatchClass {
-    int n, P, Q, nvariants;
+    int n, P, Q, nvariants, def;  // def: the variant used until autotuned
     ListVariant v[3];
 };
 
@@ -181,7 +187,7 @@ constexpr BatchClass wave_class() {
     using T0 = detail::WaveAssembly<2, N, P, Q, 3, false, detail::LuTune<0, 0>>;
     using T1 = detail::WaveAssembly<2, N, P, Q, 3, false, detail::LuTune<2, 1>>;
     using T2 = detail::WaveAssembly<2, N, P, Q, 3, false, detail::LuTune<1, 1>>;
-    return {N, P, Q, 3,
+    return {N, P, Q, 3, 1,
             {{"bc=readlane pv=shfl", &launch_list_wave<T0, 1>},
              {"bc=half pv=dpp", &launch_list_wave<T1, 1>},
              {"bc=lds pv=dpp", &launch_list_wave<T2, 1>}}};
@@ -191,7 +197,10 @@ const BatchClass kClasses[kMaxClasses] = {
     wave_class<13, 2, 3>(),
     wave_class<21, 2, 3>(),
     wave_class<35, 3, 5>(),
-    {56, 4, 7, 1, {{"workgroup nw=2", &launch_list_group<GP56>}, {nullptr, nullptr}, {nullptr, nullptr}}},
+    {56, 4, 7, 2, 0,
+     {{"blocked lu fp64 mfma", &launch_list_group<TP56, TP56::threads>},
+      {"workgroup nw=2", &launch_list_group<GP56, 128>},
+      {nullptr, nullptr}}},
 };
 
 }  // namespace
@@ -256,7 +265,8 @@ hipError_t enqueue(const rbfa_batch* b, int m, hipStream_t st) {
     hipLaunchKernelGGL(k_zero, dim3(1), dim3(64), 0, st, b->d_tickets, kMaxClasses);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_persistent, dim3(b->persistent_grid), dim3(128), 0, st, q, b->ia, b->x,
+    hipLaunchKernelGGL(k_persistent, dim3(b->persistent_grid), dim3(kPersistentThreads), 0, st, q,
+                       b->ia, b->x,
                        b->y, b->ja, out, b->info, b->status);
     return hipGetLastError();
 }
@@ -312,10 +322,11 @@ RBFA_API int32_t rbfa_batch_create(int32_t nclasses, const rbfa_plan_desc* class
     b->nclasses = nclasses;
     for (int u = 0; u < nclasses; ++u) {
         b->cls[u] = cls[idx[u]];
-        b->variant[u] = kClasses[b->cls[u]].nvariants > 1 ? 1 : 0;
+        b->variant[u] = kClasses[b->cls[u]].def;
     }
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persistent, 128, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persistent, kPersistentThreads,
+                                                     0) != hipSuccess ||
         per_cu < 1)
         per_cu = 1;
     b->persistent_grid = per_cu * prop.multiProcessorCount;
