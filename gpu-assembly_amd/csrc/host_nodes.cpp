@@ -386,6 +386,8 @@ RBFA_API const char* rbfa_status_string(int32_t status) {
         case RBFA_ERR_HIP: return "HIP runtime error";
         case RBFA_ERR_NOMEM: return "out of memory";
         case RBFA_ERR_NODEVICE: return "no gfx950 device";
+        case RBFA_ERR_FORMAT: return "malformed input file";
+        case RBFA_ERR_IO: return "file I/O error";
         default: return "unknown status";
     }
 }

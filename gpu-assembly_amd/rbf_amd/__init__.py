@@ -27,7 +27,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(_HERE), "librbf_amd.so")
 
-OK, ERR_ARG, ERR_UNSUPPORTED, ERR_HIP, ERR_NOMEM, ERR_NODEVICE = range(6)
+OK, ERR_ARG, ERR_UNSUPPORTED, ERR_HIP, ERR_NOMEM, ERR_NODEVICE, ERR_FORMAT, ERR_IO = range(8)
 OPS_DIFF, OPS_INTERP = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
 STAT_STEPS_MASK, STAT_FALLBACK = 0x03, 0x04
@@ -99,6 +99,10 @@ def lib():
         "rbfa_apply_plan_destroy": (i32, [P]),
         "rbfa_apply_plan_stats": (i32, [P, P]),
         "rbfa_apply_plan_run": (i32, [P, i32, P, P, P, P]),
+        "rbfa_last_error": (ctypes.c_char_p, []),
+        "rbfa_read_points": (i32, [ctypes.c_char_p, i32, P, P, P]),
+        "rbfa_read_graph": (i32, [ctypes.c_char_p, i32, P, P, P, P, i64]),
+        "rbfa_write_matrix_market": (i32, [ctypes.c_char_p, i64, i64, P, P, P, i32]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -406,6 +410,48 @@ class Batch:
         _check(lib().rbfa_batch_autotune(self._h, ctypes.c_void_p(stream), int(reps), ms),
                "rbfa_batch_autotune")
         return list(ms)
+
+
+def _io_check(status, what):
+    if status != OK:
+        raise RbfError(status, f"{what}: {lib().rbfa_last_error().decode()}")
+
+
+def read_points(path):
+    """x, y of a .points file (the reference's read_points, rbf_io.h:223-236)."""
+    n = ctypes.c_int32()
+    p = os.fsencode(path)
+    _io_check(lib().rbfa_read_points(p, 0, ctypes.byref(n), None, None), "rbfa_read_points")
+    x, y = np.empty(n.value), np.empty(n.value)
+    _io_check(lib().rbfa_read_points(p, n.value, ctypes.byref(n), _ptr(x), _ptr(y)),
+              "rbfa_read_points")
+    return x, y
+
+
+def read_graph(path, k=0):
+    """(ia, ja) of a .graph file, 0-based (the reference's read_graph_csr,
+    rbf_io.h:649-707); k > 0 demands rows of exactly k entries."""
+    n, nnz = ctypes.c_int32(), ctypes.c_int64()
+    p = os.fsencode(path)
+    _io_check(lib().rbfa_read_graph(p, int(k), ctypes.byref(n), ctypes.byref(nnz), None, None, 0),
+              "rbfa_read_graph")
+    ia = np.empty(n.value + 1, np.int32)
+    ja = np.empty(nnz.value, np.int32)
+    _io_check(lib().rbfa_read_graph(p, int(k), ctypes.byref(n), ctypes.byref(nnz), _ptr(ia),
+                                    _ptr(ja), ja.size), "rbfa_read_graph")
+    return ia, ja
+
+
+def write_matrix_market(path, ia, ja, a=None, cols=None, csr_base=0):
+    """Matrix Market dump of a CSR operator (the reference's
+    write_matrix_market, rbf_io.h:782-812); a=None writes the pattern."""
+    ia, ja = _i32(ia), _i32(ja)
+    rows = ia.size - 1
+    a = None if a is None else _f64(a)
+    _io_check(lib().rbfa_write_matrix_market(os.fsencode(path), rows,
+                                             rows if cols is None else int(cols), _ptr(ia),
+                                             _ptr(ja), None if a is None else _ptr(a),
+                                             int(csr_base)), "rbfa_write_matrix_market")
 
 
 def apply(ja, w, u, y, k=None, ia=None, bnd=None, stream=None):

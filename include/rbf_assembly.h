@@ -45,7 +45,9 @@ enum {
     RBFA_ERR_UNSUPPORTED = 2, /* no kernel instantiated for this shape       */
     RBFA_ERR_HIP = 3,         /* a HIP runtime call failed                   */
     RBFA_ERR_NOMEM = 4,       /* host or device allocation failed            */
-    RBFA_ERR_NODEVICE = 5     /* no usable gfx950 device                     */
+    RBFA_ERR_NODEVICE = 5,    /* no usable gfx950 device                     */
+    RBFA_ERR_FORMAT = 6,      /* malformed input file (rbfa_last_error says where) */
+    RBFA_ERR_IO = 7           /* a file could not be opened, read or written  */
 };
 
 /* ---- operator sets and precision modes --------------------------------- */
@@ -252,6 +254,36 @@ RBFA_API int32_t rbfa_batch_autotune(rbfa_batch* batch, void* stream, int32_t re
 RBFA_API int32_t rbfa_apply(int32_t nrows, int32_t k, const int32_t* ia, const int32_t* ja,
                             int32_t nops, double* const* w, const double* u, double* const* y,
                             int32_t nbnd, const int32_t* bnd, void* stream);
+
+/* ------------------------------------------------------------ file I/O
+ * The reference's file formats (rbf_io.h), host arrays only.
+ *
+ * rbfa_read_points replaces read_points (rbf_io.h:223-236): the point count
+ * on the first line, then "x y" per line. Call with cap = 0 for the count
+ * alone, then with cap >= n.
+ *
+ * rbfa_read_graph replaces read_graph_csr (rbf_io.h:649-707): header
+ * "n nnz", then one line of 0-based stencil indices per node; k > 0
+ * demands exactly k entries per row, k = 0 accepts ragged rows. Every row
+ * needs an entry, indices lie in [0, n) and appear once per row, the entry
+ * count matches nnz, nothing follows the n-th row. Call with ia = ja = NULL
+ * for n and nnz, then with ia (n+1) and ja (cap >= nnz).
+ *
+ * rbfa_write_matrix_market replaces write_matrix_market (rbf_io.h:782-812):
+ * "%%MatrixMarket matrix coordinate real general" (or "pattern" when
+ * a == NULL), "rows cols nnz", then 1-based "i j a_ij" lines in CSR order,
+ * each value the shortest text that reads back exactly; csr_base is the
+ * base of ia / ja (0 or 1).
+ *
+ * On RBFA_ERR_FORMAT / RBFA_ERR_IO, rbfa_last_error (per thread) names the
+ * file and the problem, as the reference's exception text does. */
+RBFA_API const char* rbfa_last_error(void);
+RBFA_API int32_t rbfa_read_points(const char* path, int32_t cap, int32_t* n, double* x, double* y);
+RBFA_API int32_t rbfa_read_graph(const char* path, int32_t k, int32_t* n, int64_t* nnz,
+                                 int32_t* ia, int32_t* ja, int64_t cap);
+RBFA_API int32_t rbfa_write_matrix_market(const char* path, int64_t rows, int64_t cols,
+                                          const int32_t* ia, const int32_t* ja, const double* a,
+                                          int32_t csr_base);
 
 /* Planned operator application: the same mv_op (src/main.cpp:59-66) for a
  * structure applied many times (the reference applies it once per solver

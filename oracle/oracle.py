@@ -74,6 +74,8 @@ def ref():
         L.ref_renumber_stencils.argtypes = [i32, i32, P, P]
         L.rbf_hilbert_keys.argtypes = [P, P, P, P, P, P]
         L.rbf_morton_keys.argtypes = [P, P, P, P, P, P]
+        L.ref_read_graph.argtypes = [ctypes.c_char_p, i32, P, P, P, P]
+        L.ref_write_mm.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int64, P, P, P]
         _r = L
     return _r
 

@@ -9,6 +9,8 @@
 //   ref_hilbert_order       hilbert_order            rbf_reorder.h:222-229
 //   ref_morton_order        morton_order             rbf_reorder.h:212-220
 //   ref_renumber_stencils   renumber_stencils        rbf_reorder.h:278-288
+//   ref_read_graph          read_graph_csr           rbf_io.h:649-707
+//   ref_write_mm            write_matrix_market      rbf_io.h:782-812
 //   rbf_morton_keys / rbf_hilbert_keys are the Fortran bind(c) entry
 //   points themselves (rbf_ordering.F90:31-67), linked in unchanged.
 #include <cstdint>
@@ -16,6 +18,7 @@
 #include <span>
 #include <vector>
 
+#include "rbf_io.h"
 #include "rbf_nodeset.h"
 #include "rbf_reorder.h"
 #include "rbf_spatial.h"
@@ -59,4 +62,35 @@ int ref_renumber_stencils(int32_t n, int32_t k, const int32_t* perm, int32_t* ja
     return 0;
 }
 
+
+// Sizes (ia == NULL) or contents of a .graph file. Valid files only: the
+// reference's reader ends the process (exit(1)) on a malformed one.
+int ref_read_graph(const char* path, int32_t k, int32_t* n, int64_t* nnz, int32_t* ia,
+                   int32_t* ja) {
+    try {
+        auto [ria, rja] = rbf::io::read_graph_csr<int32_t>(path, k);
+        *n = static_cast<int32_t>(ria.size() - 1);
+        *nnz = static_cast<int64_t>(rja.size());
+        if (ia) {
+            std::memcpy(ia, ria.data(), ria.size() * sizeof(int32_t));
+            std::memcpy(ja, rja.data(), rja.size() * sizeof(int32_t));
+        }
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
+int ref_write_mm(const char* path, int64_t rows, int64_t cols, const int32_t* ia,
+                 const int32_t* ja, const double* a) {
+    try {
+        if (a)
+            rbf::io::write_matrix_market<double, int32_t>(path, rows, cols, ia, ja, a, 0);
+        else
+            rbf::io::write_matrix_market_pattern<int32_t>(path, rows, cols, ia, ja, 0);
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
 }  // extern "C"

@@ -11,6 +11,14 @@ code, compiled from its sources by oracle/Makefile (oracle/_ref/libref.so):
   digests.json       SHA-256 of the Hilbert order and NodeSet::stencils(35)
                      of the 1,048,576-node cloud of the benchmark (1024^2,
                      seed 2, renumbered by hilbert_order first)
+  mm_small.npz, mm_small.mtx, mm_small_pattern.mtx
+                     a small ragged CSR operator (values chosen to exercise
+                     the shortest round-trip number text: 0.1, 1/3, the
+                     smallest subnormal, 1e300, ...) and the reference's
+                     write_matrix_market output for it (rbf_io.h:782-823)
+
+cartesian_32_21.{points,graph} and poisson_32_21.{points,graph} are copies
+of the reference's data files (data/), kept as input fixtures.
 
 The node clouds themselves come from rbfa_gen_jittered_grid; the fixture
 files store the inputs they were made from, so the tests check the inputs
@@ -42,6 +50,19 @@ def fortran_keys(fn, x, y, ndiv):
     keys = np.empty(x.size, np.int64)
     fn(ctypes.byref(n), O._p(x), O._p(y), O._p(bb), ctypes.byref(nd), O._p(keys))
     return keys
+
+
+def mm_small_inputs():
+    """12 ragged rows over 40 columns, seeded; special values first."""
+    rng = np.random.default_rng(11)
+    lens = rng.integers(1, 9, 12)
+    ia = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    ja = np.concatenate([np.sort(rng.choice(40, l, replace=False)) for l in lens]).astype(np.int32)
+    a = rng.standard_normal(ia[-1]) * 10.0 ** rng.integers(-12, 12, ia[-1])
+    special = [0.1, 1.0 / 3.0, 5e-324, 1e300, -2.5e-300, 123456789.0, 2.0, 1.0000000000000002,
+               np.pi, -0.0, 1e-5, 6.02214076e23]
+    a[:len(special)] = special
+    return ia, ja, a
 
 
 def main():
@@ -80,6 +101,13 @@ def main():
     }
     with open(os.path.join(HERE, "digests.json"), "w") as f:
         json.dump(dig, f, indent=1)
+
+    ia, ja, a = mm_small_inputs()
+    np.savez_compressed(os.path.join(HERE, "mm_small.npz"), ia=ia, ja=ja, a=a)
+    for name, vals in (("mm_small.mtx", a), ("mm_small_pattern.mtx", None)):
+        rc = ref.ref_write_mm(os.fsencode(os.path.join(HERE, name)), ia.size - 1, 40, O._p(ia),
+                              O._p(ja), None if vals is None else O._p(vals))
+        assert rc == 0
     print("golden vectors written")
 
 
