@@ -29,6 +29,8 @@
 
 #include <omp.h>
 
+#include "curve_keys.h"
+#include "knn_grid.h"
 #include "rbf_assembly.h"
 
 namespace {
@@ -38,63 +40,12 @@ namespace {
 // ---------------------------------------------------------------------------
 
 inline int64_t morton_key2(double px, double py, const rbfa_bbox2& bb, int ndiv) {
-    double x0 = bb.xmin, y0 = bb.ymin, x1 = bb.xmax, y1 = bb.ymax;
-    int64_t z = 0;
-    for (int i = 0; i < ndiv; ++i) {
-        z <<= 2;
-        const double cx = x0 + 0.5 * (x1 - x0);
-        const double cy = y0 + 0.5 * (y1 - y0);
-        const bool right = px > cx, up = py > cy;
-        z += (right ? 1 : 0) + (up ? 2 : 0);
-        if (right) x0 = cx; else x1 = cx;
-        if (up) y0 = cy; else y1 = cy;
-    }
-    return z;
+    return rbf_amd_internal::morton_key2(px, py, bb.xmin, bb.ymin, bb.xmax, bb.ymax, ndiv);
 }
-
-// Hilbert state machine. kDigit[rot][q] is the 2-bit digit emitted for
-// quadrant q (bit 0: right half, bit 1: upper half) in orientation rot;
-// kNext[rot][q] is the orientation of that sub-square. These are the
-// tables of hidx (rbf_ordering.F90:164-181), indexed [rot][q] here.
-constexpr int kDigit[4][4] = {{0, 3, 1, 2}, {2, 3, 1, 0}, {2, 1, 3, 0}, {0, 1, 3, 2}};
-constexpr int kNext[4][4] = {{3, 1, 0, 0}, {1, 0, 1, 2}, {2, 2, 3, 1}, {0, 3, 2, 3}};
-
 inline int64_t hilbert_key2(double px, double py, const rbfa_bbox2& bb, int ndiv) {
-    double x0 = bb.xmin, y0 = bb.ymin, x1 = bb.xmax, y1 = bb.ymax;
-    int64_t h = 0;
-    int rot = 0;
-    for (int i = 0; i < ndiv; ++i) {
-        h <<= 2;
-        const double cx = x0 + 0.5 * (x1 - x0);
-        const double cy = y0 + 0.5 * (y1 - y0);
-        const bool right = px > cx, up = py > cy;
-        const int q = (right ? 1 : 0) + (up ? 2 : 0);
-        h += kDigit[rot][q];
-        rot = kNext[rot][q];
-        if (right) x0 = cx; else x1 = cx;
-        if (up) y0 = cy; else y1 = cy;
-    }
-    return h;
+    return rbf_amd_internal::hilbert_key2(px, py, bb.xmin, bb.ymin, bb.xmax, bb.ymax, ndiv);
 }
-
-inline int64_t morton_key3(double px, double py, double pz, const double* bb, int ndiv) {
-    double lo[3] = {bb[0], bb[1], bb[2]}, hi[3] = {bb[3], bb[4], bb[5]};
-    const double p[3] = {px, py, pz};
-    int64_t z = 0;
-    for (int i = 0; i < ndiv; ++i) {
-        z <<= 3;
-        for (int d = 0; d < 3; ++d) {
-            const double c = lo[d] + 0.5 * (hi[d] - lo[d]);
-            if (p[d] > c) {
-                z += int64_t(1) << d;
-                lo[d] = c;
-            } else {
-                hi[d] = c;
-            }
-        }
-    }
-    return z;
-}
+using rbf_amd_internal::morton_key3;
 
 rbfa_bbox2 bbox_of(int32_t n, const double* x, const double* y) {
     rbfa_bbox2 b{x[0], y[0], x[0], y[0]};
@@ -189,32 +140,9 @@ Grid<D> build_grid(int32_t n, const double* const* X, int k) {
             hi[d] = std::max(hi[d], X[d][i]);
         }
     }
-    double vol = 1.0;
-    int nonflat = 0;
-    for (int d = 0; d < D; ++d)
-        if (hi[d] > g.lo[d]) {
-            vol *= hi[d] - g.lo[d];
-            ++nonflat;
-        }
-    // about max(2, k/8) points per cell
-    const double per_cell = std::max(2.0, k / 8.0);
-    double cell = nonflat ? std::pow(vol * per_cell / n, 1.0 / nonflat) : 1.0;
-    if (!(cell > 0) || !std::isfinite(cell)) cell = 1.0;
+    g.cell = rbf_amd_internal::grid_params<D>(g.lo, hi, n, k, g.dims);
     long long total = 1;
-    for (int d = 0; d < D; ++d) {
-        const double ext = hi[d] - g.lo[d];
-        long long m = ext > 0 ? static_cast<long long>(ext / cell) + 1 : 1;
-        m = std::min<long long>(m, D == 2 ? 8192 : 512);
-        g.dims[d] = static_cast<int>(m);
-        total *= m;
-    }
-    // cell edge: the largest extent per cell across axes, so every axis is covered
-    double c = 0;
-    for (int d = 0; d < D; ++d) {
-        const double ext = hi[d] - g.lo[d];
-        if (ext > 0) c = std::max(c, ext / g.dims[d]);
-    }
-    g.cell = c > 0 ? c * (1.0 + 1e-12) : 1.0;
+    for (int d = 0; d < D; ++d) total *= g.dims[d];
     std::vector<int32_t> cell_of(n);
     g.start.assign(static_cast<size_t>(total) + 1, 0);
     for (int32_t i = 0; i < n; ++i) {

@@ -100,6 +100,10 @@ def lib():
         "rbfa_apply_plan_stats": (i32, [P, P]),
         "rbfa_apply_plan_run": (i32, [P, i32, P, P, P, P]),
         "rbfa_last_error": (ctypes.c_char_p, []),
+        "rbfa_knn_stencils_device": (i32, [i32, i32, P, P, P, i32, P, P]),
+        "rbfa_curve_keys_device": (i32, [i32, i32, i32, P, P, P, P, i32, P, P]),
+        "rbfa_order_by_keys_device": (i32, [i32, P, i32, P, P]),
+        "rbfa_gather_f64_device": (i32, [i32, P, P, P, P]),
         "rbfa_read_points": (i32, [ctypes.c_char_p, i32, P, P, P]),
         "rbfa_read_graph": (i32, [ctypes.c_char_p, i32, P, P, P, P, i64]),
         "rbfa_write_matrix_market": (i32, [ctypes.c_char_p, i64, i64, P, P, P, i32]),
@@ -410,6 +414,58 @@ class Batch:
         _check(lib().rbfa_batch_autotune(self._h, ctypes.c_void_p(stream), int(reps), ms),
                "rbfa_batch_autotune")
         return list(ms)
+
+
+def _stream(stream):
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream if stream is None else stream)
+
+
+def _dptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def knn_stencils_device(k, x, y, z=None, stream=None):
+    """rbfa_knn_stencils on the GPU: x, y (, z) float64 CUDA tensors; returns
+    the int32 CUDA tensor ja (n*k), identical to knn_stencils."""
+    import torch
+    n = x.numel()
+    ja = torch.empty(n * int(k), dtype=torch.int32, device=x.device)
+    _check(lib().rbfa_knn_stencils_device(n, 2 if z is None else 3, _dptr(x), _dptr(y), _dptr(z),
+                                          int(k), _dptr(ja), _stream(stream)),
+           "rbfa_knn_stencils_device")
+    return ja
+
+
+def curve_keys_device(x, y, z=None, curve="hilbert", ndiv=16, bbox=None, stream=None):
+    """Morton / Hilbert keys (2-D) or octree Morton keys (3-D) on the GPU."""
+    import torch
+    n = x.numel()
+    keys = torch.empty(n, dtype=torch.int64, device=x.device)
+    bb = None if bbox is None else np.ascontiguousarray(bbox, dtype=np.float64)
+    _check(lib().rbfa_curve_keys_device(n, 2 if z is None else 3, 1 if curve == "hilbert" else 0,
+                                        _dptr(x), _dptr(y), _dptr(z),
+                                        None if bb is None else _ptr(bb), int(ndiv), _dptr(keys),
+                                        _stream(stream)), "rbfa_curve_keys_device")
+    return keys
+
+
+def order_by_keys_device(keys, key_bits=62, stream=None):
+    """Stable order of non-negative int64 keys on the GPU (gather form)."""
+    import torch
+    perm = torch.empty(keys.numel(), dtype=torch.int32, device=keys.device)
+    _check(lib().rbfa_order_by_keys_device(keys.numel(), _dptr(keys), int(key_bits), _dptr(perm),
+                                           _stream(stream)), "rbfa_order_by_keys_device")
+    return perm
+
+
+def gather_device(perm, src, stream=None):
+    """dst[i] = src[perm[i]] on the GPU."""
+    import torch
+    dst = torch.empty_like(src)
+    _check(lib().rbfa_gather_f64_device(src.numel(), _dptr(perm), _dptr(src), _dptr(dst),
+                                        _stream(stream)), "rbfa_gather_f64_device")
+    return dst
 
 
 def _io_check(status, what):

@@ -147,6 +147,34 @@ RBFA_API int32_t rbfa_gen_jittered_grid(int32_t dim, int32_t m, double jitter, u
 RBFA_API int32_t rbfa_gen_graded_disk(int32_t n_interior, int32_t n_boundary, uint64_t seed,
                                       double* x, double* y, int32_t* flag);
 
+/* ---- the node layer on the device (device pointers, stream-ordered) -----
+ * Same results as the host functions above, bit for bit, for the same
+ * points: the same keys, the same stable order, the same stencils (the
+ * device search uses the host search's bucket grid, ring order and
+ * stopping rule). Each call synchronises `stream` before it returns and
+ * allocates its temporaries itself (hipMalloc / hipFree).
+ *
+ * rbfa_knn_stencils_device: rbfa_knn_stencils (NodeSet::stencils,
+ * rbf_nodeset.h:155-181) for k <= 128, one wave per node.
+ * rbfa_curve_keys_device: dim 2, curve 0 = Morton / 1 = Hilbert
+ * (rbf_ordering.F90 zidx / hidx, rbf_reorder.h:198-210), bbox =
+ * {xmin, ymin, xmax, ymax}; dim 3, curve 0 = octree Morton, bbox =
+ * {xmin, ymin, zmin, xmax, ymax, zmax}; bbox NULL: the points' own box.
+ * rbfa_order_by_keys_device: stable order of keys in [0, 2^key_bits)
+ * (detail::order_by_keys, rbf_reorder.h:183-189); perm is the gather form.
+ * rbfa_gather_f64_device: dst[i] = src[perm[i]] (Permutation::permute,
+ * rbf_reorder.h:108-115), src != dst. */
+RBFA_API int32_t rbfa_knn_stencils_device(int32_t n, int32_t dim, const double* x,
+                                          const double* y, const double* z, int32_t k,
+                                          int32_t* ja, void* stream);
+RBFA_API int32_t rbfa_curve_keys_device(int32_t n, int32_t dim, int32_t curve, const double* x,
+                                        const double* y, const double* z, const double* bbox,
+                                        int32_t ndiv, int64_t* keys, void* stream);
+RBFA_API int32_t rbfa_order_by_keys_device(int32_t n, const int64_t* keys, int32_t key_bits,
+                                           int32_t* perm, void* stream);
+RBFA_API int32_t rbfa_gather_f64_device(int32_t n, const int32_t* perm, const double* src,
+                                        double* dst, void* stream);
+
 /* ======================================================================
  * Device assembly
  * ====================================================================== */

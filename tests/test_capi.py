@@ -75,6 +75,18 @@ def test_apply_plan_validates_structure_before_touching_a_device():
     assert e.value.status == R.ERR_ARG                           # a row wider than 2048
 
 
+def test_device_node_layer_argument_checks():
+    L = R.lib()
+    # rejected before any device call: k > 128, k > n, bad dim, bad ndiv
+    assert L.rbfa_knn_stencils_device(10, 2, 1, 1, None, 129, 1, None) == R.ERR_ARG
+    assert L.rbfa_knn_stencils_device(10, 2, 1, 1, None, 11, 1, None) == R.ERR_ARG
+    assert L.rbfa_knn_stencils_device(10, 4, 1, 1, None, 5, 1, None) == R.ERR_ARG
+    assert L.rbfa_curve_keys_device(10, 2, 1, 1, 1, None, None, 32, 1, None) == R.ERR_ARG
+    assert L.rbfa_curve_keys_device(10, 3, 1, 1, 1, 1, None, 8, 1, None) == R.ERR_ARG
+    assert L.rbfa_order_by_keys_device(10, 1, 64, 1, None) == R.ERR_ARG
+    assert L.rbfa_gather_f64_device(10, 1, 2, 2, None) == R.ERR_ARG
+
+
 def test_assemble_rejects_null_plan():
     assert R.lib().rbfa_assemble(None, 10, None, None, None, None, None, None, None, None, None,
                                  None, None) == R.ERR_ARG
