@@ -22,6 +22,10 @@ Other workloads (--workload), reported in the same format:
   disk_2m_uneven   graded unit disk, 2,000,000 interior + 8,000 boundary nodes,
                    n in {13, 21, 35, 56} by size class, one ragged graph,
                    captured hipGraph (per-class launches or one persistent launch)
+  nodes_1m_n35     the node layer on the device for the headline cloud: Hilbert
+                   keys, stable order, coordinate gathers and n = 35 kNN
+                   stencils (bit-identical to the reference's hilbert_order +
+                   NodeSet::stencils); one step = the whole pipeline
   apply_1m_n35     operator application on the headline workload's operators:
                    y_c = W_c u for Laplacian/d/dx/d/dy, boundary rows (the
                    outermost ring of grid cells) set to the identity; one
@@ -139,7 +143,68 @@ WORKLOADS = {
     "cube_2m_n80": ("exp3", 1 << 17),
     "disk_2m_uneven": ("sincos", 1 << 22),
     "apply_1m_n35": ("sincos", 1 << 20),
+    "nodes_1m_n35": ("sincos", 1 << 20),
 }
+
+
+def run_nodes(args, R, torch, dev):
+    """Device node layer: keys -> stable order -> gathers -> kNN (n = 35)."""
+    import hashlib
+    x, y = R.jittered_grid(2, 1024, 0.25, seed=args.seed)
+    xd, yd = torch.from_numpy(x).to(dev), torch.from_numpy(y).to(dev)
+    out = {}
+
+    def step():
+        keys = R.curve_keys_device(xd, yd, curve="hilbert", ndiv=16)
+        perm = R.order_by_keys_device(keys, key_bits=32)
+        xs, ys = R.gather_device(perm, xd), R.gather_device(perm, yd)
+        out["ja"] = R.knn_stencils_device(35, xs, ys)
+        out["perm"] = perm
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    ja = out["ja"].cpu().numpy()
+    perm = out["perm"].cpu().numpy()
+    log(f"[nodes] {ms:.2f} ms/step for {x.size} nodes")
+    res = {"ms": ms, "ja_sha256": hashlib.sha256(ja.tobytes()).hexdigest(),
+           "perm_sha256": hashlib.sha256(perm.tobytes()).hexdigest()}
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        cores = len(os.sched_getaffinity(0))
+        if O.ref_available():
+            L = O.ref()
+            t0 = time.perf_counter()
+            order = np.empty(x.size, np.int32)
+            L.ref_hilbert_order(x.size, O._p(x), O._p(y), 16, O._p(order))
+            xs, ys = np.ascontiguousarray(x[order]), np.ascontiguousarray(y[order])
+            jr = np.empty(x.size * 35, np.int32)
+            L.ref_stencils_nanoflann(x.size, O._p(xs), O._p(ys), 35, O._p(jr))
+            cpu_s = time.perf_counter() - t0
+            cpu = {"value": x.size / cpu_s, "unit": "nodes/s", "cores": cores, "kind": "reference",
+                   "sample": f"all {x.size} nodes: the reference's hilbert_order + "
+                             f"NodeSet::stencils(35) (nanoflann, compiled from its sources), "
+                             f"{cpu_s:.2f} s"}
+            res["identical_to_reference"] = bool(np.array_equal(order, perm) and
+                                                 np.array_equal(jr, ja))
+        else:
+            t0 = time.perf_counter()
+            p = R.hilbert_order(x, y, 16)
+            jh = R.knn_stencils(35, p.permute(x), p.permute(y))
+            cpu_s = time.perf_counter() - t0
+            cpu = {"value": x.size / cpu_s, "unit": "nodes/s", "cores": cores, "kind": "port",
+                   "sample": f"all {x.size} nodes: this library's host node layer, {cpu_s:.2f} s"}
+            res["identical_to_reference"] = bool(np.array_equal(p.map(), perm) and
+                                                 np.array_equal(jh, ja))
+        log(f"[nodes] CPU {cpu_s * 1e3:.0f} ms, identical {res['identical_to_reference']}")
+    return x.size, res, cpu
+
 
 
 def copy_calibration(torch, dev):
@@ -387,6 +452,31 @@ def main():
     name = args.workload
     fname, default_sample = WORKLOADS[name]
     sample = args.cpu_sample or default_sample
+    if name == "nodes_1m_n35":
+        nn, res, cpu = run_nodes(args, R, torch, dev)
+        ms = res["ms"]
+        # algorithmic HBM bytes per node: coordinates read (16) and gathered (16 read,
+        # 16 written), keys (8 written, 8 read), order (4 written), stencils (35 x 4)
+        row_bytes = 16 + 32 + 16 + 4 + 140
+        gbs = row_bytes * nn / (ms * 1e-3) / 1e9
+        line = {
+            "metric": "nodes/s through the device node layer (Hilbert order + kNN n=35), 1M nodes",
+            "value": nn / (ms * 1e-3), "unit": "nodes/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64 distances, int32/int64 keys",
+            "data": "synthetic (seeded jittered grid)",
+            "config": {"workload": "nodes_1m_n35: 1024x1024 jittered unit square -> Hilbert keys "
+                                   "(ndiv 16) -> stable order -> gathers -> kNN n=35",
+                       "nodes": nn},
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                         "frac": gbs / PEAK_HBM_GBS, "bytes_per_node": row_bytes,
+                         "traffic": None,
+                         "note": "latency-bound (per-query candidate insertion), not bandwidth"},
+            "cpu_baseline": cpu,
+            "parity": res,
+        }
+        print(json.dumps(line), flush=True)
+        return
     if name == "apply_1m_n35":
         W, res, cpu = run_apply(args, R, workloads, torch, dev)
         if args.profile_copy:

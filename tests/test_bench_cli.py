@@ -10,10 +10,10 @@ sys.path.insert(0, ROOT)
 
 def test_every_workload_has_a_handler():
     import bench
-    for fn in ("run_fixed", "oracle_fixed", "run_disk", "oracle_disk", "run_apply", "main"):
+    for fn in ("run_fixed", "oracle_fixed", "run_disk", "oracle_disk", "run_apply", "run_nodes", "main"):
         assert callable(getattr(bench, fn)), fn
     assert {"square_4k_n21", "square_1m_n35", "square_1m_n56", "cube_2m_n80", "disk_2m_uneven",
-            "apply_1m_n35"} <= set(bench.WORKLOADS)
+            "apply_1m_n35", "nodes_1m_n35"} <= set(bench.WORKLOADS)
 
 
 def test_more_than_one_gpu_is_refused():
