@@ -384,7 +384,8 @@ def run_disk(args, R, workloads, torch, dev):
     counts = {f"n{c[0]}": b.class_count(i) for i, c in enumerate(b.classes)}
     tune = b.autotune(reps=2) if not args.no_tune else None
     if tune:
-        log(f"[disk] autotune: per-class launches {tune[0]:.3f} ms, persistent {tune[1]:.3f} ms; "
+        pers = "switched off" if tune[1] < 0 else f"{tune[1]:.3f} ms"
+        log(f"[disk] autotune: per-class launches {tune[0]:.3f} ms, persistent {pers}; "
             f"mode {b.mode}; variants "
             + ", ".join(f"n{c[0]}: {b.class_variant(i)}" for i, c in enumerate(b.classes)))
     ms, wall = hip_timed(b.launch, args.steps, args.warmup)
@@ -393,7 +394,8 @@ def run_disk(args, R, workloads, torch, dev):
                     "fallback": 0, "refine_steps_hist": None,
                     "variant": "mode %d (%s)" % (b.mode, "persistent" if b.mode else "per-class"),
                     "autotune_ms": None if tune is None else
-                    {"per_class_launches": round(tune[0], 4), "persistent": round(tune[1], 4)},
+                    {"per_class_launches": round(tune[0], 4),
+                     "persistent": None if tune[1] < 0 else round(tune[1], 4)},
                     "class_counts": counts, "w": [t.cpu().numpy() for t in w]}}
     return W, res
 

@@ -116,6 +116,12 @@ using GP56 = detail::GroupAssembly<2, 56, 4, 7, 3, false, 2>;
 using TP56 = detail::TileAssembly<2, 56, 4, 7>;
 using PK = Persistent<TP56, EP35, EP21, EP13>;
 constexpr int kPersistentThreads = TP56::threads;
+// The persistent launch (mode 1) is switched off: a run of it faulted the
+// GPU (memory aperture violation) after the single-wave engine's fill and
+// back substitution changed, with no cause found in its code or in the
+// compiled kernel. Mode 0 (one launch per class, the same engines, all in
+// one captured graph) is unaffected. The kernel stays compiled.
+constexpr bool kPersistentEnabled = false;
 
 __global__ void __launch_bounds__(kPersistentThreads)
 k_persistent(Queues q, const int* __restrict__ ia, const double* __restrict__ x,
@@ -184,13 +190,13 @@ struct BatchClass {
 
 template <int N, int P, int Q>
 constexpr BatchClass wave_class() {
-    using T0 = detail::WaveAssembly<2, N, P, Q, 3, false, detail::LuTune<0, 0>>;
+    using T0 = detail::WaveAssembly<2, N, P, Q, 3, false, detail::LuTune<2, 2, 0, 0, 1>>;
     using T1 = detail::WaveAssembly<2, N, P, Q, 3, false, detail::LuTune<2, 1>>;
-    using T2 = detail::WaveAssembly<2, N, P, Q, 3, false, detail::LuTune<1, 1>>;
-    return {N, P, Q, 3, 1,
-            {{"bc=readlane pv=shfl", &launch_list_wave<T0, 1>},
+    using T2 = detail::WaveAssembly<2, N, P, Q, 3, false, detail::LuTune<0, 2, 1, 1, 1>>;
+    return {N, P, Q, 3, 0,
+            {{"bc=half pv=dpp-bcast pin", &launch_list_wave<T0, 1>},
              {"bc=half pv=dpp", &launch_list_wave<T1, 1>},
-             {"bc=lds pv=dpp", &launch_list_wave<T2, 1>}}};
+             {"bc=readlane pv=dpp-bcast pin fill=pairs bs=lds", &launch_list_wave<T2, 1>}}};
 }
 
 const BatchClass kClasses[kMaxClasses] = {
@@ -405,6 +411,7 @@ RBFA_API int32_t rbfa_batch_class_count(const rbfa_batch* b, int32_t u) {
 
 RBFA_API int32_t rbfa_batch_set_mode(rbfa_batch* b, int32_t mode) {
     if (!b || (mode != 0 && mode != 1)) return RBFA_ERR_ARG;
+    if (mode == 1 && !kPersistentEnabled) return RBFA_ERR_UNSUPPORTED;
     b->mode = mode;
     return RBFA_OK;
 }
@@ -456,15 +463,16 @@ RBFA_API int32_t rbfa_batch_autotune(rbfa_batch* b, void* stream, int32_t reps, 
             }
         }
     }
-    // 2. the two captured pipelines
+    // 2. the captured pipelines (the persistent one only while enabled;
+    //    its time is reported as -1 otherwise)
     drop_graphs(b);
-    float ms_mode[2] = {0.f, 0.f};
-    for (int m = 0; m < 2 && rc == RBFA_OK; ++m) {
+    float ms_mode[2] = {0.f, -1.f};
+    for (int m = 0; m < (kPersistentEnabled ? 2 : 1) && rc == RBFA_OK; ++m) {
         rc = capture(b, m);
         if (rc == RBFA_OK) rc = time_it([&] { return hipGraphLaunch(b->exec[m], st); }, ms_mode[m]);
     }
     if (rc == RBFA_OK) {
-        b->mode = ms_mode[1] < ms_mode[0] ? 1 : 0;
+        b->mode = kPersistentEnabled && ms_mode[1] < ms_mode[0] ? 1 : 0;
         if (ms_out) {
             ms_out[0] = ms_mode[0];
             ms_out[1] = ms_mode[1];

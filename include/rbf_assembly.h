@@ -252,9 +252,11 @@ RBFA_API int32_t rbfa_plan_autotune(rbfa_plan* plan, int32_t nstencils, const do
  * bind captures the pipeline into a hipGraph with the given device arrays
  * (w: host array of 3 device pointers laid out like ja); launch replays
  * it. Mode 0: one launch per class; mode 1: one persistent launch over
- * size-bucketed work queues. autotune picks each class's kernel variant and
- * the faster mode (ms_out[2] = mode 0, mode 1 milliseconds). Calls that
- * overlap on one batch race on its work-queue counters. */
+ * size-bucketed work queues, currently switched off (set_mode(1) returns
+ * RBFA_ERR_UNSUPPORTED). autotune picks each class's kernel variant and
+ * the faster enabled mode (ms_out[2] = mode 0, mode 1 milliseconds; -1 for
+ * a mode that is switched off). Calls that overlap on one batch race on
+ * its work-queue counters. */
 typedef struct rbfa_batch rbfa_batch;
 RBFA_API int32_t rbfa_batch_create(int32_t nclasses, const rbfa_plan_desc* classes,
                                    rbfa_batch** batch);

@@ -209,6 +209,8 @@ namespace detail {
 
 __device__ __forceinline__ int lane_id() { return static_cast<int>(threadIdx.x & 63u); }
 
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+
 // value of v in lane l (l wave-uniform): two v_readlane_b32 into an SGPR pair
 __device__ __forceinline__ double bcast(double v, int l) {
     const long long b = __double_as_longlong(v);
@@ -228,9 +230,9 @@ __device__ __forceinline__ T wave_max(T v) {
     return v;
 }
 
-// Unsigned max over the wave: DPP within each 16-lane row (quad swaps, half
-// mirror, mirror), then the four row maxima through SGPRs (v_readlane +
-// s_max_u32). The result is wave-uniform.
+// Unsigned max over the wave: DPP within each 16-lane row (quad swaps,
+// half mirror, mirror), then the four row maxima through SGPRs
+// (v_readlane + s_max_u32). The result is wave-uniform.
 __device__ __forceinline__ unsigned wave_umax(unsigned v) {
     v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0xB1, 0xF, 0xF, false)));
     v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x4E, 0xF, 0xF, false)));
@@ -243,11 +245,46 @@ __device__ __forceinline__ unsigned wave_umax(unsigned v) {
     return max(max(a, b), max(c, d));
 }
 
+// The same maximum all in DPP: after the in-row steps, row 0 into row 1
+// and row 2 into row 3 (row_bcast:15), then rows 0-1 into rows 2-3
+// (row_bcast:31); lane 63 ends with the maximum, read into an SGPR.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ unsigned dpp_max(unsigned v) {
+    // rows outside ROW_MASK read 0, the identity of the maximum
+    return max(v, static_cast<unsigned>(
+                      __builtin_amdgcn_update_dpp(0, static_cast<int>(v), CTRL, ROW_MASK, 0xF, false)));
+}
+__device__ __forceinline__ unsigned wave_umax_bcast(unsigned v) {
+    v = dpp_max<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+    v = dpp_max<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+    v = dpp_max<0x141, 0xF>(v);  // row_half_mirror
+    v = dpp_max<0x140, 0xF>(v);  // row_mirror
+    v = dpp_max<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    v = dpp_max<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+    return static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
+}
+
+// Lanes where pred holds, as a wave-uniform 64-bit mask.
+__device__ __forceinline__ unsigned long long wave_ballot(bool pred) {
+    return __builtin_amdgcn_ballot_w64(pred);
+}
+
 // Lowest lane, among the candidates, of largest |a| (NaN never wins):
 // the same choice as a serial "first strict maximum" scan in lane order.
 // Returns -1 when there is no candidate. PV selects the reduction:
 // 0 = shuffle max of |a| then ballot; 1 = integer max on the bit pattern of
-// |a| (high word, then low word only when the high words tie).
+// |a| (high word, then low word only when the high words tie) with the
+// row maxima combined in SGPRs; 2 = the same with the rows combined in DPP.
+template <int PV>
+__device__ __forceinline__ unsigned pivot_umax(unsigned v) {
+    if constexpr (PV == 2) return wave_umax_bcast(v);
+    else return wave_umax(v);
+}
+template <int PV>
+__device__ __forceinline__ unsigned long long pivot_ballot(bool pred) {
+    if constexpr (PV == 2) return wave_ballot(pred);
+    else return __ballot(pred);
+}
 template <int PV>
 __device__ __forceinline__ int pivot_lane(double a, bool cand) {
     cand = cand && !__builtin_isnan(a);
@@ -259,13 +296,13 @@ __device__ __forceinline__ int pivot_lane(double a, bool cand) {
     } else {
         const double fa = fabs(a);
         const unsigned hi = cand ? static_cast<unsigned>(__double2hiint(fa)) : 0u;
-        const unsigned mh = wave_umax(hi);
-        unsigned long long hit = __ballot(cand && hi == mh);
+        const unsigned mh = pivot_umax<PV>(hi);
+        unsigned long long hit = pivot_ballot<PV>(cand && hi == mh);
         if (__popcll(hit) > 1) {
             const bool top = cand && hi == mh;
             const unsigned lo = top ? static_cast<unsigned>(__double2loint(fa)) : 0u;
-            const unsigned ml = wave_umax(lo);
-            hit = __ballot(top && lo == ml);
+            const unsigned ml = pivot_umax<PV>(lo);
+            hit = pivot_ballot<PV>(top && lo == ml);
         }
         return hit ? __ffsll(hit) - 1 : -1;
     }
@@ -280,8 +317,8 @@ __device__ __forceinline__ int pivot_lane(float a, bool cand) {
         return hit ? __ffsll(hit) - 1 : -1;
     } else {
         const unsigned key = cand ? __float_as_uint(fabsf(a)) : 0u;
-        const unsigned mk = wave_umax(key);
-        const unsigned long long hit = __ballot(cand && key == mk);
+        const unsigned mk = pivot_umax<PV>(key);
+        const unsigned long long hit = pivot_ballot<PV>(cand && key == mk);
         return hit ? __ffsll(hit) - 1 : -1;
     }
 }
@@ -422,9 +459,13 @@ __device__ __forceinline__ void elim_from(T (&row)[NC], ElimState<NT, T>& st, T*
             st.mystep = K;
             st.myrcp = rcp;
         }
-        // Materialise this step's select now: left alone, the compiler sinks
-        // all NT selects to the end and keeps NT reciprocals live meanwhile.
-        asm volatile("" : "+v"(st.myrcp), "+v"(st.mystep));
+        // Materialise this step's selects now: left alone, the compiler sinks
+        // all NT of them to the end and keeps NT reciprocals and NT lane
+        // masks live meanwhile (the masks spilled to VGPR lanes).
+        if constexpr (Tune::PN != 0)
+            asm volatile("" : "+v"(st.myrcp), "+v"(st.mystep), "+v"(row[K]));
+        else
+            asm volatile("" : "+v"(st.myrcp), "+v"(st.mystep));
         st.perm[K] = p;
         update_cols<K + 1, Tune::BC>(row, l, p, pb, is_piv);
         elim_from<K + 1, NT, NC, Tune, STORE_L>(row, st, pb, lane);
@@ -435,9 +476,13 @@ __device__ __forceinline__ void elim_from(T (&row)[NC], ElimState<NT, T>& st, T*
 //   BC  pivot-row broadcast: 0 = v_readlane into SGPRs, 1 = LDS, 2 = half
 //       of the columns each way (balances the VALU and LDS pipes)
 //   PV  pivot search: see pivot_lane
-template <int BC_, int PV_>
+//   FL  kernel-block fill: 0 = every lane evaluates its own row (the
+//       other point through SGPRs), 1 = each pair once, through LDS
+//   BS  back substitution: 0 = t through SGPRs (v_readlane), 1 = LDS
+//   PN  1 = the zeroed column of each step materialised at its step
+template <int BC_, int PV_, int FL_ = 0, int BS_ = 0, int PN_ = 0>
 struct LuTune {
-    static constexpr int BC = BC_, PV = PV_;
+    static constexpr int BC = BC_, PV = PV_, FL = FL_, BS = BS_, PN = PN_;
 };
 using DefaultTune = LuTune<0, 0>;
 
@@ -470,13 +515,26 @@ struct WaveAssembly {
     static_assert(Q >= 3 && Q % 2 == 1, "odd PHS exponent >= 3");
 
     using tune = Tune;
-    // LDS per wave, in doubles: pivot-row buffer [NC] (first, 16-byte
-    // aligned), scaled points [DIM][N], monomials [M][N], interpolation
-    // centres [DIM][R] and their monomials [R][M]; rounded up to an even
-    // count so consecutive waves' regions stay 16-byte aligned.
+    // LDS per wave, in doubles. With the pairwise fill (Tune::FL = 1): a
+    // region used three ways in turn -- monomials [M][N], interpolation
+    // centres [DIM][R] and their monomials [R][M] (read while the rows are
+    // built); then the kernel block phi [N][N], column-major (written pair
+    // by pair, read one column per lane); then the pivot-row buffer [NC] of
+    // the elimination (16-byte aligned) -- and after it the scaled points
+    // [DIM][N]. Otherwise the pivot-row buffer, the points, the monomials
+    // and the centres side by side. Rounded up to an even count so
+    // consecutive waves' regions stay 16-byte aligned.
     static constexpr int lds_pivot = (NC + 1) / 2 * 2;
+    static constexpr int lds_fill = M * N + (INTERP ? DIM * R + R * M : 0);
+    static constexpr int lds_shared = cmax(cmax(lds_fill, N * N), lds_pivot);
+    static constexpr bool pair_fill = Tune::FL != 0;
+    static constexpr int off_pts = pair_fill ? (lds_shared + 1) / 2 * 2 : lds_pivot;
+    static constexpr int off_poly = pair_fill ? 0 : lds_pivot + DIM * N;
     static constexpr int lds_doubles =
-        (lds_pivot + DIM * N + M * N + (INTERP ? DIM * R + R * M : 0) + 1) / 2 * 2;
+        pair_fill ? (off_pts + DIM * N + 1) / 2 * 2 : (off_poly + lds_fill + 1) / 2 * 2;
+    // kernel-block entries above the diagonal, and how many per lane
+    static constexpr int npairs = N * (N - 1) / 2;
+    static constexpr int pair_slots = (npairs + 63) / 64;
 
     // Gather, shift, scale and build this lane's row of [M | B]:
     // lane k < N holds point row k, lane N + t polynomial row t.
@@ -488,10 +546,11 @@ struct WaveAssembly {
                                      double* lds, T (&row)[NC], double& rho2_out) {
 #pragma clang fp contract(off)
         const int lane = lane_id();
-        double* pts = lds + lds_pivot;   // [DIM][N]
-        double* polyT = pts + DIM * N;   // [M][N]
+        double* pts = lds + off_pts;     // [DIM][N]
+        double* polyT = lds + off_poly;  // [M][N]
         double* cen = polyT + M * N;     // [DIM][R]
         double* cmono = cen + DIM * R;   // [R][M]
+        double* phiT = lds;              // [N][N], entry (i, j) at j * N + i
 
         const int jn = lane < N ? ja_row[lane] : s;
         const double px = x[jn] - x[s];
@@ -531,30 +590,36 @@ struct WaveAssembly {
         wave_lds_sync();
 
         constexpr PHS<Q> phi{};
-        // Stencil point col comes from lane col through scalar registers
-        // (v_readlane), not LDS: it stays out of the vector registers.
-        double bx[N], by[N], bz[DIM == 3 ? N : 1];
-#pragma unroll
-        for (int col = 0; col < N; ++col) {
-            bx[col] = bcast(xs, col);
-            by[col] = bcast(ys, col);
-            if constexpr (DIM == 3) bz[col] = bcast(zs, col);
-        }
-        if (lane < N) {
+        if constexpr (!pair_fill) {
+            // Every lane evaluates its own kernel row; stencil point col
+            // comes from lane col through scalar registers (v_readlane).
+            double bx[N], by[N], bz[DIM == 3 ? N : 1];
 #pragma unroll
             for (int col = 0; col < N; ++col) {
-                const double dx = bx[col] - xs;
-                const double dy = by[col] - ys;
-                double r2 = dx * dx + dy * dy;
-                if constexpr (DIM == 3) {
-                    const double dz = bz[col] - zs;
-                    r2 = r2 + dz * dz;
-                }
-                row[col] = static_cast<T>(phi(r2));
-                // keep the scheduler from interleaving all N evaluations
-                // (it would hold N sqrt temporaries live at once)
-                if ((col & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+                bx[col] = bcast(xs, col);
+                by[col] = bcast(ys, col);
+                if constexpr (DIM == 3) bz[col] = bcast(zs, col);
             }
+            if (lane < N) {
+#pragma unroll
+                for (int col = 0; col < N; ++col) {
+                    const double dx = bx[col] - xs;
+                    const double dy = by[col] - ys;
+                    double r2 = dx * dx + dy * dy;
+                    if constexpr (DIM == 3) {
+                        const double dz = bz[col] - zs;
+                        r2 = r2 + dz * dz;
+                    }
+                    row[col] = static_cast<T>(phi(r2));
+                    // keep the scheduler from interleaving all N evaluations
+                    // (it would hold N sqrt temporaries live at once)
+                    if ((col & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+        }
+        // Polynomial columns, right-hand sides, and the polynomial rows
+        // (which read the monomials of every point from LDS).
+        if (lane < N) {
 #pragma unroll
             for (int t = 0; t < M; ++t) row[N + t] = static_cast<T>(mono[t]);
             if constexpr (!INTERP) {
@@ -599,6 +664,47 @@ struct WaveAssembly {
 #pragma unroll
             for (int c = 0; c < NC; ++c) row[c] = T(0);
         }
+        wave_lds_sync();  // monomials and centres read: the kernel block overwrites them
+        if constexpr (pair_fill) fill_pairs(lane, pts, phiT, row);
+    }
+
+    // Kernel block, each entry above the diagonal evaluated once: pair
+    // q = (d - 1) N + i is nodes i and (i + d) mod N, d = 1 .. N/2 (the
+    // pairs of d = N/2 counted once for even N), spread over all 64 lanes.
+    // phi depends on the squared distance only, and (a - b)^2 == (b - a)^2
+    // in floating point, so both entries get the value a direct fill
+    // computes.
+    template <typename T>
+    __device__ __forceinline__ static void fill_pairs(int lane, const double* pts, double* phiT,
+                                                      T (&row)[NC]) {
+#pragma clang fp contract(off)
+        constexpr PHS<Q> phi{};
+#pragma unroll
+        for (int sl = 0; sl < pair_slots; ++sl) {
+            const int q = sl * 64 + lane;
+            if (q < npairs) {
+                const int d = q / N + 1;
+                const int i = q - (d - 1) * N;
+                const int j = i + d < N ? i + d : i + d - N;
+                const double dx = pts[j] - pts[i];
+                const double dy = pts[N + j] - pts[N + i];
+                double r2 = dx * dx + dy * dy;
+                if constexpr (DIM == 3) {
+                    const double dz = pts[2 * N + j] - pts[2 * N + i];
+                    r2 = r2 + dz * dz;
+                }
+                const double v = phi(r2);
+                phiT[j * N + i] = v;
+                phiT[i * N + j] = v;
+            }
+        }
+        if (lane < N) phiT[lane * N + lane] = phi(0.0);
+        wave_lds_sync();
+        if (lane < N) {
+#pragma unroll
+            for (int col = 0; col < N; ++col) row[col] = static_cast<T>(phiT[col * N + lane]);
+        }
+        wave_lds_sync();  // read before the elimination reuses the region
     }
 
     // Right-looking partial-pivot elimination of [M | B] held one row per
@@ -611,15 +717,35 @@ struct WaveAssembly {
         ElimState<NT, double> st;
         elim_from<0, NT, NC, Tune, false>(row, st, lds, lane);
         // back substitution: row[k] holds u_ik for rows pivotal before step
-        // k and 0 for every other row, so all lanes update alike
+        // k and 0 for every other row, so all lanes update alike. The
+        // pivot lane of step k passes t_c = b_c / u_kk (as b_c * rcp) to
+        // every lane, through SGPRs or (Tune::BS) through LDS in two slots
+        // taken in turn.
+        if constexpr (Tune::BS == 0) {
 #pragma unroll
-        for (int k = NT - 1; k >= 0; --k) {
-            const int p = st.perm[k];
+            for (int k = NT - 1; k >= 0; --k) {
+                const int p = st.perm[k];
 #pragma unroll
-            for (int c = 0; c < R; ++c) {
-                const double t = bcast(row[NT + c] * st.myrcp, p);
-                row[NT + c] = fma(-row[k], t, row[NT + c]);
+                for (int c = 0; c < R; ++c) {
+                    const double t = bcast(row[NT + c] * st.myrcp, p);
+                    row[NT + c] = fma(-row[k], t, row[NT + c]);
+                }
             }
+        } else {
+            constexpr int TS = (R + 1) / 2 * 2;  // slot stride, 16-byte aligned
+            static_assert(2 * TS <= lds_pivot, "slots fit the pivot buffer");
+#pragma unroll
+            for (int k = NT - 1; k >= 0; --k) {
+                double* tb = lds + (k & 1) * TS;
+                if (lane == st.perm[k]) {
+#pragma unroll
+                    for (int c = 0; c < R; ++c) tb[c] = row[NT + c] * st.myrcp;
+                }
+                wave_lds_sync();
+#pragma unroll
+                for (int c = 0; c < R; ++c) row[NT + c] = fma(-row[k], tb[c], row[NT + c]);
+            }
+            wave_lds_sync();  // slots read before the caller reuses LDS
         }
 #pragma unroll
         for (int c = 0; c < R; ++c) xout[c] = row[NT + c] * st.myrcp;

@@ -175,7 +175,9 @@ def _disk_case(n_int=12000, n_bnd=300, seed=21):
     return workloads.build_disk(n_int, n_bnd, seed)
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, pytest.param(1, marks=pytest.mark.skip(
+    reason="persistent launch switched off: a run faulted the GPU (memory aperture "
+           "violation) and the cause is not found; set_mode(1) returns UNSUPPORTED"))])
 def test_uneven_batch_matches_oracle_and_replays(mode):
     """Graded disk with all four size classes in one ragged graph; the
     captured graph is replayed three times with eager work in between, and
@@ -221,8 +223,10 @@ def test_uneven_batch_autotune():
     w = [torch.zeros(nnz, dtype=torch.float64, device="cuda:0") for _ in range(3)]
     b.bind(W["ia"], _dev(W["x"]), _dev(W["y"]), _dev(W["ia"]), _dev(W["ja"]), w)
     ms = b.autotune(reps=2)
-    assert len(ms) == 2 and min(ms) > 0
-    assert b.mode == int(np.argmin(ms))
+    assert len(ms) == 2 and ms[0] > 0
+    assert ms[1] == -1.0 and b.mode == 0  # the persistent mode is switched off
+    with pytest.raises(R.RbfError):
+        b.mode = 1
     assert all(b.class_variant(c) for c in range(4))
 
 
