@@ -360,6 +360,16 @@ def oracle_fixed(W, res, n, dim, sample):
     cpu = {"value": cnt / cpu_s, "unit": "stencils/s", "cores": cores, "kind": "port",
            "sample": f"{cnt} contiguous stencils (from {s0}) of the same workload, FP64 oracle "
                      f"(gather, fill, partial-pivot LU, solve, scatter), {cpu_s:.2f} s"}
+    # the same oracle on one core, over a sixteenth of the sample
+    c1 = max(1, cnt // 16)
+    t0 = time.perf_counter()
+    O.assemble(dim, n, W["P"], W["Q"], W["x"], W["y"], W["ja"], z=W["z"], s0=s0, count=c1,
+               nthreads=1)
+    one_s = time.perf_counter() - t0
+    cpu["single_core"] = {"value": c1 / one_s, "unit": "stencils/s", "cores": 1,
+                          "sample": f"{c1} contiguous stencils (from {s0}), {one_s:.2f} s"}
+    log(f"CPU oracle: {cnt / cpu_s:.0f} stencils/s on {cores} threads, "
+        f"{c1 / one_s:.0f} on one")
     errs = {}
     for form, d in res.items():
         wg = [a[s0 * n:(s0 + cnt) * n] for a in d["w"]]
@@ -566,6 +576,9 @@ def main():
 
     if args.profile_copy:
         copy_calibration(torch, dev)
+    peaks = R.measure_peaks()
+    log("measured peaks: FP64 VALU %.1f TFLOP/s, FP64 MFMA %.1f TFLOP/s, HBM copy %.0f GB/s"
+        % (peaks["fp64_valu_tflops"], peaks["fp64_mfma_tflops"], peaks["hbm_copy_gbs"]))
 
     kernel_tag = "assemble_%s_%s" % (head, "n35_1m" if name == HEADLINE else name)
     line = {
@@ -587,7 +600,9 @@ def main():
         "roofline": {"bound": "mfma", "peak_kind": "fp64 (vector = matrix on MI355X)",
                      "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS,
-                     "flop_per_stencil": flop, "traffic": read_traffic(kernel_tag)},
+                     "flop_per_stencil": flop, "traffic": read_traffic(kernel_tag),
+                     "peak_measured": {k: round(v, 2) for k, v in peaks.items()},
+                     "frac_of_measured_fp64_valu": achieved_tflops / peaks["fp64_valu_tflops"]},
         "cpu_baseline": cpu,
         "forms": extra,
         "host_prep_s": W["host_s"],

@@ -63,6 +63,7 @@ def lib():
     i32, i64, u64, dbl = ctypes.c_int32, ctypes.c_int64, ctypes.c_uint64, ctypes.c_double
     sig = {
         "rbfa_version": (ctypes.c_char_p, []),
+        "rbfa_measure_peaks": (i32, [P, P]),
         "rbfa_status_string": (ctypes.c_char_p, [i32]),
         "rbfa_morton_keys": (i32, [i32, P, P, P, i32, P]),
         "rbfa_hilbert_keys": (i32, [i32, P, P, P, i32, P]),
@@ -508,6 +509,14 @@ def write_matrix_market(path, ia, ja, a=None, cols=None, csr_base=0):
                                              rows if cols is None else int(cols), _ptr(ia),
                                              _ptr(ja), None if a is None else _ptr(a),
                                              int(csr_base)), "rbfa_write_matrix_market")
+
+
+def measure_peaks(stream=None):
+    """Peaks of the GPU at hand, by microbenchmark: FP64 vector FMA and FP64
+    MFMA TFLOP/s, HBM copy GB/s (the roofline peaks, re-measured)."""
+    out = (ctypes.c_double * 3)()
+    _check(lib().rbfa_measure_peaks(out, _stream(stream)), "rbfa_measure_peaks")
+    return {"fp64_valu_tflops": out[0], "fp64_mfma_tflops": out[1], "hbm_copy_gbs": out[2]}
 
 
 def apply(ja, w, u, y, k=None, ia=None, bnd=None, stream=None):

@@ -79,6 +79,13 @@ typedef struct rbfa_bbox2 {
 } rbfa_bbox2;
 
 RBFA_API const char* rbfa_version(void);
+
+/* Re-measured peaks of the GPU at hand (no counterpart in the reference;
+ * the benchmark prices its roofline against them beside the datasheet
+ * figures): out[0] FP64 vector FMA TFLOP/s, out[1] FP64 MFMA
+ * (v_mfma_f64_16x16x4f64) TFLOP/s, out[2] HBM copy GB/s (1 GiB each way).
+ * Allocates and frees its own buffers; synchronises the stream. */
+RBFA_API int32_t rbfa_measure_peaks(double* out, void* stream);
 RBFA_API const char* rbfa_status_string(int32_t status);
 
 /* ======================================================================

@@ -186,3 +186,14 @@ def test_assemble_on_reference_data_files_and_dump(stem, tmp_path):
     import scipy.sparse as sp
     B = sp.csr_matrix((wref[0], ja, ia), shape=(x.size, x.size))
     assert (A != B).nnz == 0
+
+
+def test_measured_peaks_are_plausible():
+    """The roofline peaks re-measured on the GPU: FP64 vector and matrix
+    FMA throughput and HBM copy bandwidth, each below its MI355X datasheet
+    figure (78.6 / 78.6 TFLOP/s, 8 TB/s) and above a quarter of it."""
+    pk = R.measure_peaks()
+    print(pk)
+    assert 78.6 / 4 < pk["fp64_valu_tflops"] <= 78.6 * 1.05
+    assert 78.6 / 4 < pk["fp64_mfma_tflops"] <= 78.6 * 1.05
+    assert 2000 < pk["hbm_copy_gbs"] <= 8000 * 1.05

@@ -36,6 +36,10 @@ def test_library_exports_every_declared_symbol():
     assert names <= exported
 
 
+def test_measure_peaks_rejects_null_output():
+    assert R.lib().rbfa_measure_peaks(None, None) == R.ERR_ARG
+
+
 def test_version_and_status_strings():
     assert b"gfx950" in R.lib().rbfa_version()
     assert R.status_string(R.ERR_UNSUPPORTED) == "unsupported stencil shape"
