@@ -32,7 +32,7 @@ k_assemble_group_mixed(int nstencils, const double* __restrict__ x, const double
 // The FP64 solve of the stencils the mixed kernel could not converge,
 // persistent over the fallback list (the blocked-LU engine, four waves).
 template <class G>
-__global__ void __launch_bounds__(G::threads)
+__global__ void __launch_bounds__(G::threads, G::occupancy)
 k_group_fallback(const double* __restrict__ x, const double* __restrict__ y,
                  const double* __restrict__ z, const int* __restrict__ ja,
                  detail::ValueOut<double, G::nrhs> out, int* __restrict__ info,

@@ -12,7 +12,7 @@ namespace rbf_amd_internal {
 using namespace rbf_operators;
 
 template <class E>
-__global__ void __launch_bounds__(E::threads)
+__global__ void __launch_bounds__(E::threads, E::occupancy)
 k_assemble_tile(int nstencils, const double* __restrict__ x, const double* __restrict__ y,
                 const double* __restrict__ z, const int* __restrict__ ja,
                 detail::ValueOut<double, E::nrhs> out, int* __restrict__ info,

@@ -55,8 +55,8 @@ k_list_wave(int count, const int* __restrict__ order, const int* __restrict__ ia
     if (status && detail::lane_id() == 0) status[s] = 0;
 }
 
-template <class G, int THREADS>
-__global__ void __launch_bounds__(THREADS)
+template <class G, int THREADS, int OCC>
+__global__ void __launch_bounds__(THREADS, OCC)
 k_list_group(int count, const int* __restrict__ order, const int* __restrict__ ia,
              const double* __restrict__ x, const double* __restrict__ y,
              const double* __restrict__ z, const int* __restrict__ ja, Out3 out,
@@ -171,10 +171,10 @@ hipError_t launch_list_wave(const ListArgs& a) {
                        a.count, a.order, a.ia, a.x, a.y, a.z, a.ja, a.out, a.info, a.status);
     return hipGetLastError();
 }
-template <class G, int THREADS>
+template <class G, int THREADS, int OCC = 1>
 hipError_t launch_list_group(const ListArgs& a) {
     if (a.count == 0) return hipSuccess;
-    hipLaunchKernelGGL((k_list_group<G, THREADS>), dim3(xcd_grid(a.count)), dim3(THREADS), 0, a.stream,
+    hipLaunchKernelGGL((k_list_group<G, THREADS, OCC>), dim3(xcd_grid(a.count)), dim3(THREADS), 0, a.stream,
                        a.count, a.order, a.ia, a.x, a.y, a.z, a.ja, a.out, a.info, a.status);
     return hipGetLastError();
 }
@@ -204,7 +204,7 @@ const BatchClass kClasses[kMaxClasses] = {
     wave_class<21, 2, 3>(),
     wave_class<35, 3, 5>(),
     {56, 4, 7, 2, 0,
-     {{"blocked lu fp64 mfma", &launch_list_group<TP56, TP56::threads>},
+     {{"blocked lu fp64 mfma", &launch_list_group<TP56, TP56::threads, TP56::occupancy>},
       {"workgroup nw=2", &launch_list_group<GP56, 128>},
       {nullptr, nullptr}}},
 };

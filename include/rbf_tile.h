@@ -88,6 +88,12 @@ struct TileAssembly {
     static constexpr int lds_doubles = off_int + (NT + MP + 2) / 2;
     static constexpr int threads = 64 * W;
     static_assert(lds_doubles * 8 <= 65536, "static LDS limit");
+    // Waves per SIMD to ask the register allocator for (the kernels'
+    // __launch_bounds__): as many four-wave workgroups as the CU's 160 KB
+    // of LDS holds, at most three (168 registers). Three workgroups per CU
+    // beat two even where 168 registers spill a few values to scratch
+    // (71x71: 115.7 ms against 147.0 ms for 1M stencils).
+    static constexpr int occupancy = (160 * 1024) / (lds_doubles * 8) >= 3 ? 3 : 2;
 
     // ------------------------------------------------------------------ fill
     // Entry (i, j) of [M | B | 0] from the scaled points and monomials in
