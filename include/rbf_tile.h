@@ -44,7 +44,7 @@ namespace detail {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-template <int DIM, int N, int P, int Q, int W = 4>
+template <int DIM, int N, int P, int Q, int W = 4, bool PANEL_PRIO = true>
 struct TileAssembly {
     static constexpr int M = npoly_dim(DIM, P);
     static constexpr int NT = N + M;
@@ -342,6 +342,10 @@ struct TileAssembly {
     __device__ __forceinline__ static void factor(dbl4 (&tile)[RT], int q, double* lds, int lane) {
         int* ints = reinterpret_cast<int*>(lds + off_int);
         double* X = lds + off_x;
+        // The panel is the stencil's critical path: while it runs, this wave
+        // takes issue priority on its SIMD over the trailing updates of the
+        // other waves there (wave priority only; the stream's is untouched).
+        if constexpr (PANEL_PRIO) __builtin_amdgcn_s_setprio(3);
         tile_to_rows(tile, X, lane);
         wave_lds_sync();
         double p0[16], p1[16];
@@ -368,6 +372,7 @@ struct TileAssembly {
         wave_lds_sync();
         if constexpr (STORE_L) rows_to_tile(tile, X, lane);
         if (lane == 0 && info && ints[int_info] == 0) ints[int_info] = info;
+        if constexpr (PANEL_PRIO) __builtin_amdgcn_s_setprio(0);
     }
 
     // ------------------------------------------------------------ trailing
