@@ -91,7 +91,7 @@ struct Variant {
 constexpr int kVariants = 6;
 
 // The tuning space per shape (detail::LuTune): pivot-row broadcast
-// (readlane / LDS / half and half), pivot search (shuffle / DPP with the
+// (readlane / LDS / half or a third of the columns through LDS), pivot search (shuffle / DPP with the
 // rows combined in SGPRs / all in DPP), kernel-block fill (own row / each
 // pair once), back-substitution broadcast (readlane / LDS), and whether each
 // step's zeroed column is materialised at its step. Variant 0, the default
@@ -99,10 +99,10 @@ constexpr int kVariants = 6;
 template <template <class, int> class L, template <class> class Eng>
 struct VariantSet {
     static constexpr Variant list[kVariants] = {
+        {"bc=third pv=dpp-bcast pin fill=pairs", &L<Eng<detail::LuTune<4, 2, 1, 0, 1>>, 1>::run},
         {"bc=half pv=dpp-bcast pin", &L<Eng<detail::LuTune<2, 2, 0, 0, 1>>, 1>::run},
         {"bc=half pv=dpp", &L<Eng<detail::LuTune<2, 1>>, 1>::run},
         {"bc=readlane pv=shfl", &L<Eng<detail::LuTune<0, 0>>, 1>::run},
-        {"bc=lds pv=dpp-bcast pin", &L<Eng<detail::LuTune<1, 2, 0, 0, 1>>, 1>::run},
         {"bc=half pv=dpp-bcast pin fill=pairs", &L<Eng<detail::LuTune<2, 2, 1, 0, 1>>, 1>::run},
         {"bc=readlane pv=dpp-bcast pin fill=pairs bs=lds",
          &L<Eng<detail::LuTune<0, 2, 1, 1, 1>>, 1>::run},

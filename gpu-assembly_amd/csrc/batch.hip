@@ -190,11 +190,11 @@ struct BatchClass {
 
 template <int N, int P, int Q>
 constexpr BatchClass wave_class() {
-    using T0 = detail::WaveAssembly<2, N, P, Q, 3, false, detail::LuTune<2, 2, 0, 0, 1>>;
+    using T0 = detail::WaveAssembly<2, N, P, Q, 3, false, detail::LuTune<4, 2, 1, 0, 1>>;
     using T1 = detail::WaveAssembly<2, N, P, Q, 3, false, detail::LuTune<2, 1>>;
     using T2 = detail::WaveAssembly<2, N, P, Q, 3, false, detail::LuTune<0, 2, 1, 1, 1>>;
     return {N, P, Q, 3, 0,
-            {{"bc=half pv=dpp-bcast pin", &launch_list_wave<T0, 1>},
+            {{"bc=third pv=dpp-bcast pin fill=pairs", &launch_list_wave<T0, 1>},
              {"bc=half pv=dpp", &launch_list_wave<T1, 1>},
              {"bc=readlane pv=dpp-bcast pin fill=pairs bs=lds", &launch_list_wave<T2, 1>}}};
 }

@@ -332,10 +332,12 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // Pivot-row columns routed through LDS (the rest go lane-to-SGPR):
-// MODE 0 none, 1 all, 2 every other group of VW columns (VW = 16 bytes).
+// MODE 0 none, 1 all, 2 every other group of VW columns (VW = 16 bytes),
+// 3 two groups of every three, 4 one group of every three.
 template <int MODE, int VW>
 __device__ constexpr bool via_lds(int j) {
-    return MODE == 1 || (MODE == 2 && ((j / VW) & 1));
+    return MODE == 1 || (MODE == 2 && ((j / VW) & 1)) || (MODE == 3 && (j / VW) % 3 != 0) ||
+           (MODE == 4 && (j / VW) % 3 == 2);
 }
 
 // Rank-1 update of columns J0..NC-1 of every lane's row with the pivot
@@ -474,7 +476,8 @@ __device__ __forceinline__ void elim_from(T (&row)[NC], ElimState<NT, T>& st, T*
 
 // Knobs of the wave LU, chosen per stencil shape by the autotuner:
 //   BC  pivot-row broadcast: 0 = v_readlane into SGPRs, 1 = LDS, 2 = half
-//       of the columns each way (balances the VALU and LDS pipes)
+//       of the columns each way (balances the VALU and LDS pipes), 3 = two
+//       thirds through LDS, 4 = one third through LDS
 //   PV  pivot search: see pivot_lane
 //   FL  kernel-block fill: 0 = every lane evaluates its own row (the
 //       other point through SGPRs), 1 = each pair once, through LDS
