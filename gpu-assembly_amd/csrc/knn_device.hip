@@ -196,7 +196,56 @@ struct TopK {
             wi = __shfl(last < 64 ? i0 : i1, last & 63, 64);
         }
     }
+
+    // Compare-exchange with lane ^ stride: keep the smaller pair (take_min)
+    // or the larger one.
+    __device__ __forceinline__ static void cmpx(double& d, int& i, int stride, bool take_min) {
+        const double pd = __shfl_xor(d, stride, 64);
+        const int pi = __shfl_xor(i, stride, 64);
+        if (take_min == less_pair(pd, pi, d, i)) {
+            d = pd;
+            i = pi;
+        }
+    }
+
+    // k <= 64: merge a chunk of candidates (one per lane; lanes not kept
+    // hold (inf, INT_MAX)) with `nkeep` kept into the list at once. The
+    // chunk is sorted across the wave (bitonic), reversed against the list,
+    // the pairwise minima form a bitonic sequence holding the 64 smallest
+    // of both, and six merge stages sort it. The first min(k, count +
+    // nkeep) entries are the k smallest pairs of list and chunk -- what
+    // inserting the chunk's candidates one by one leaves, since (d, index)
+    // is a strict total order and a candidate above the k-th is never kept.
+    __device__ __forceinline__ void merge_chunk(double cd, int ci, int nkeep, int k, int lane) {
+        for (int size = 2; size <= 64; size <<= 1)
+            for (int stride = size >> 1; stride > 0; stride >>= 1)
+                cmpx(cd, ci, stride, ((lane & stride) == 0) == ((lane & size) == 0));
+        if (lane >= count) {
+            a0 = __builtin_inf();
+            i0 = INT_MAX;
+        }
+        const double rd = __shfl(cd, 63 - lane, 64);
+        const int ri = __shfl(ci, 63 - lane, 64);
+        if (less_pair(rd, ri, a0, i0)) {
+            a0 = rd;
+            i0 = ri;
+        }
+        for (int stride = 32; stride > 0; stride >>= 1) cmpx(a0, i0, stride, (lane & stride) == 0);
+        count = count + nkeep < k ? count + nkeep : k;
+        if (lane >= count) {
+            a0 = __builtin_inf();
+            i0 = INT_MAX;
+        }
+        if (count == k) {
+            wd = __shfl(a0, k - 1, 64);
+            wi = __shfl(i0, k - 1, 64);
+        }
+    }
 };
+
+// Chunks with at least this many candidates below the current k-th are
+// merged at once rather than inserted one by one.
+constexpr int kMergeMin = 8;
 
 // Candidates in slots [sb, se), 64 at a time.
 template <int D>
@@ -216,7 +265,12 @@ __device__ __forceinline__ void scan_slots(int sb, int se, const GridDev& g, con
             }
             idx = g.order[slot];
         }
-        unsigned long long m = __ballot(valid && (t.count < k || less_pair(acc, idx, t.wd, t.wi)));
+        const bool keep = valid && (t.count < k || less_pair(acc, idx, t.wd, t.wi));
+        unsigned long long m = __ballot(keep);
+        if (k <= 64 && __popcll(m) >= kMergeMin) {
+            t.merge_chunk(keep ? acc : __builtin_inf(), keep ? idx : INT_MAX, __popcll(m), k, lane);
+            continue;
+        }
         while (m) {
             const int j = __ffsll(m) - 1;
             m &= m - 1;
