@@ -208,14 +208,20 @@ struct TopK {
         }
     }
 
-    // k <= 64: merge a chunk of candidates (one per lane; lanes not kept
-    // hold (inf, INT_MAX)) with `nkeep` kept into the list at once. The
-    // chunk is sorted across the wave (bitonic), reversed against the list,
-    // the pairwise minima form a bitonic sequence holding the 64 smallest
-    // of both, and six merge stages sort it. The first min(k, count +
-    // nkeep) entries are the k smallest pairs of list and chunk -- what
-    // inserting the chunk's candidates one by one leaves, since (d, index)
-    // is a strict total order and a candidate above the k-th is never kept.
+    // Merge a chunk of candidates (one per lane; lanes not kept hold
+    // (inf, INT_MAX)) with `nkeep` kept into the list at once. The chunk
+    // is sorted across the wave (bitonic) and reversed against the first
+    // 64 entries: the pairwise minima are the 64 smallest of both, the
+    // maxima the 64 largest, each a bitonic sequence sorted by six merge
+    // stages. The minima are the new entries 0..63 (none exceeds an entry
+    // 64.. of the list); for k > 64 the maxima are merged the same way
+    // with entries 64..127, keeping the smaller 64. The first min(k, count
+    // + nkeep) entries are then the k smallest pairs of list and chunk --
+    // what inserting the candidates one by one leaves, since (d, index) is
+    // a strict total order and a candidate above the k-th is never kept.
+    __device__ __forceinline__ static void sort_merge(double& d, int& i, int lane) {
+        for (int stride = 32; stride > 0; stride >>= 1) cmpx(d, i, stride, (lane & stride) == 0);
+    }
     __device__ __forceinline__ void merge_chunk(double cd, int ci, int nkeep, int k, int lane) {
         for (int size = 2; size <= 64; size <<= 1)
             for (int stride = size >> 1; stride > 0; stride >>= 1)
@@ -224,21 +230,45 @@ struct TopK {
             a0 = __builtin_inf();
             i0 = INT_MAX;
         }
-        const double rd = __shfl(cd, 63 - lane, 64);
-        const int ri = __shfl(ci, 63 - lane, 64);
-        if (less_pair(rd, ri, a0, i0)) {
+        double rd = __shfl(cd, 63 - lane, 64);
+        int ri = __shfl(ci, 63 - lane, 64);
+        const bool rl = less_pair(rd, ri, a0, i0);
+        double hd = rl ? a0 : rd;  // the larger of the pair
+        int hi = rl ? i0 : ri;
+        if (rl) {
             a0 = rd;
             i0 = ri;
         }
-        for (int stride = 32; stride > 0; stride >>= 1) cmpx(a0, i0, stride, (lane & stride) == 0);
+        sort_merge(a0, i0, lane);
+        if (k > 64) {
+            sort_merge(hd, hi, lane);
+            if (lane + 64 >= count) {
+                a1 = __builtin_inf();
+                i1 = INT_MAX;
+            }
+            rd = __shfl(a1, 63 - lane, 64);
+            ri = __shfl(i1, 63 - lane, 64);
+            if (less_pair(rd, ri, hd, hi)) {
+                hd = rd;
+                hi = ri;
+            }
+            sort_merge(hd, hi, lane);
+            a1 = hd;
+            i1 = hi;
+        }
         count = count + nkeep < k ? count + nkeep : k;
         if (lane >= count) {
             a0 = __builtin_inf();
             i0 = INT_MAX;
         }
+        if (lane + 64 >= count) {
+            a1 = __builtin_inf();
+            i1 = INT_MAX;
+        }
         if (count == k) {
-            wd = __shfl(a0, k - 1, 64);
-            wi = __shfl(i0, k - 1, 64);
+            const int last = k - 1;
+            wd = __shfl(last < 64 ? a0 : a1, last & 63, 64);
+            wi = __shfl(last < 64 ? i0 : i1, last & 63, 64);
         }
     }
 };
@@ -267,7 +297,7 @@ __device__ __forceinline__ void scan_slots(int sb, int se, const GridDev& g, con
         }
         const bool keep = valid && (t.count < k || less_pair(acc, idx, t.wd, t.wi));
         unsigned long long m = __ballot(keep);
-        if (k <= 64 && __popcll(m) >= kMergeMin) {
+        if (__popcll(m) >= kMergeMin) {
             t.merge_chunk(keep ? acc : __builtin_inf(), keep ? idx : INT_MAX, __popcll(m), k, lane);
             continue;
         }
