@@ -190,16 +190,34 @@ struct TileAssembly {
         const bool cb = c0 || c1;
         const unsigned long long kb = use1 ? k1 : k0;
         const unsigned hi = cb ? static_cast<unsigned>(kb >> 32) : 0u;
-        const unsigned mh = wave_umax(hi);
-        bool top = cb && hi == mh;
-        const unsigned long long any = __ballot(top);
-        if (__popcll(any) > 1) {
-            const unsigned lo = top ? static_cast<unsigned>(kb) : 0u;
-            const unsigned ml = wave_umax(lo);
-            top = top && lo == ml;
+        const unsigned mh = wave_umax_bcast(hi);
+        unsigned long long h0, h1;
+        if constexpr (occupancy == 2) {
+            // lane masks combined in SGPRs: a ballot of a plain compare
+            // needs no VALU select (2.4% on the 100x100 systems); under the
+            // 168-register cap of three workgroups per CU the two extra
+            // live masks cost more in spills than they save (71x71: +1.4%)
+            const unsigned long long mcb = __ballot(cb);
+            const unsigned long long mu1 = __ballot(use1);
+            unsigned long long top = wave_ballot(hi == mh) & mcb;
+            if (__popcll(top) > 1) {
+                const unsigned lo = ((top >> lane_id()) & 1ull) ? static_cast<unsigned>(kb) : 0u;
+                const unsigned ml = wave_umax_bcast(lo);
+                top &= wave_ballot(lo == ml);
+            }
+            h0 = top & ~mu1;
+            h1 = top & mu1;
+        } else {
+            bool top = cb && hi == mh;
+            const unsigned long long any = __ballot(top);
+            if (__popcll(any) > 1) {
+                const unsigned lo = top ? static_cast<unsigned>(kb) : 0u;
+                const unsigned ml = wave_umax_bcast(lo);
+                top = top && lo == ml;
+            }
+            h0 = __ballot(top && !use1);
+            h1 = __ballot(top && use1);
         }
-        const unsigned long long h0 = __ballot(top && !use1);
-        const unsigned long long h1 = __ballot(top && use1);
         int win;
         if (h0) {
             win = __ffsll(h0) - 1;
@@ -239,7 +257,7 @@ struct TileAssembly {
     __device__ __forceinline__ static void panel_step(double (&p0)[16], double (&p1)[16],
                                                       bool& act0, bool& act1, int k0, double* lb,
                                                       double* lds, int& info, int lane, int win,
-                                                      double rcp) {
+                                                      double rcp, int& sw, double& sr) {
         if constexpr (C < KQ) {
             int* ints = reinterpret_cast<int*>(lds + off_int);
             const int k = k0 + C;
@@ -259,10 +277,11 @@ struct TileAssembly {
                 lb[lane * S + C] = -l0;
                 if (lane + 64 < MP) lb[(lane + 64) * S + C] = -l1;
             }
-            if (lane == 0) {
-                ints[int_perm + k] = win;
-                ints[int_step + win] = k;
-                lds[off_rcp + k] = rcp;
+            // step C's pivot row and reciprocal stay in lane C until the
+            // panel ends (nothing reads them before)
+            if (lane == C) {
+                sw = win;
+                sr = rcp;
             }
             // the pivot lane's two rows' columns C+2.. to LDS, the pivot row
             // read back below (the slot is chosen by address, not by a select)
@@ -288,7 +307,7 @@ struct TileAssembly {
                 panel_update_lds<C + 2>(p0, p1, l0, l1, pr);
                 wave_lds_sync();  // reads done before the next step's row is written
                 panel_step<C + 1, KQ, STORE_L>(p0, p1, act0, act1, k0, lb, lds, info, lane, nwin,
-                                               nrcp);
+                                               nrcp, sw, sr);
             } else {
                 wave_lds_sync();
                 panel_update_lds<C + 2>(p0, p1, l0, l1, pr);
@@ -355,8 +374,15 @@ struct TileAssembly {
         int info = 0;
         double rcp0;
         const int win0 = pivot_of(p0[0], p1[0], act0, act1, 16 * q, info, rcp0);
+        int sw = 0;
+        double sr = 0.0;
         panel_step<0, KQ, STORE_L>(p0, p1, act0, act1, 16 * q, lds + off_l + (q & 1) * XB, lds,
-                                   info, lane, win0, rcp0);
+                                   info, lane, win0, rcp0, sw, sr);
+        if (lane < KQ) {
+            ints[int_perm + 16 * q + lane] = sw;
+            ints[int_step + sw] = 16 * q + lane;
+            lds[off_rcp + 16 * q + lane] = sr;
+        }
         wave_lds_sync();
         store_rows(p0, p1, X, lane);
         if constexpr (STORE_L) {
