@@ -17,8 +17,9 @@
 // above 64) and eliminates it with the wave-LU step of rbf_operators.h,
 // restricted to the panel's 16 columns: pivot = largest |a| among rows not
 // yet pivotal (first row on ties), multiplier l = a * (1 / pivot), column
-// zeroed, panel columns updated by fma(-l, u, a). It publishes -l for
-// every row, the pivot rows and their reciprocals. After one barrier every
+// zeroed, panel columns updated by fma(-l, u, a), the pivot row's values
+// read from its lane into SGPRs. It publishes -l for every row, the pivot
+// rows and their reciprocals. After one barrier every
 // wave updates its own column tiles to the right of the panel:
 //   U block: the panel's 16 pivot rows at those columns, by the same
 //            chain of fma the oracle applies to them while still active;
@@ -67,8 +68,7 @@ struct TileAssembly {
     // LDS (doubles): [fill: points, monomials | panel row block [MP][S]],
     // multiplier panels [2][MP][S] (reused as the back-substitution block
     // buffers), U staging [W][16][16], the multipliers of each panel's
-    // pivot rows in step order [2][16][16], the pivot lane's two rows [2][16],
-    // pivot reciprocals [NT], per-wave
+    // pivot rows in step order [2][16][16], pivot reciprocals [NT], per-wave
     // maxima [W], then ints: pivot row of each step [NT], step of each row
     // [MP], info.
     static constexpr int FILL = DIM * N + M * N;
@@ -80,8 +80,7 @@ struct TileAssembly {
     static constexpr int off_l = R0;
     static constexpr int off_u = off_l + 2 * XB;
     static constexpr int off_l11 = off_u + W * 256;
-    static constexpr int off_prow = off_l11 + 2 * 256;
-    static constexpr int off_rcp = off_prow + 32;
+    static constexpr int off_rcp = off_l11 + 2 * 256;
     static constexpr int off_wmax = off_rcp + (NT + 1) / 2 * 2;
     static constexpr int off_int = off_wmax + (W + 1) / 2 * 2;
     static constexpr int int_perm = 0, int_step = NT, int_info = NT + MP;
@@ -237,15 +236,17 @@ struct TileAssembly {
     }
 
     // Update panel columns J.. of both row sets with the pivot row's values
-    // u[J..] (read back from LDS at one address: a broadcast).
+    // u[J..], read from the pivot lane into SGPRs (v_readlane): the panel
+    // is latency-bound, and this is shorter than a round trip through LDS
+    // (71x71: 93.9 against 101.9 ms; 100x100: 455.8 against 470.9 ms).
     template <int J>
-    __device__ __forceinline__ static void panel_update_lds(double (&p0)[16], double (&p1)[16],
-                                                            double l0, double l1, const double* pr) {
+    __device__ __forceinline__ static void panel_update(double (&p0)[16], double (&p1)[16],
+                                                        double l0, double l1, bool ws, int wl) {
         if constexpr (J < 16) {
-            const double u = pr[J];
+            const double u = bcast(ws ? p1[J] : p0[J], wl);
             p0[J] = fma(-l0, u, p0[J]);
             p1[J] = fma(-l1, u, p1[J]);
-            panel_update_lds<J + 1>(p0, p1, l0, l1, pr);
+            panel_update<J + 1>(p0, p1, l0, l1, ws, wl);
         }
     }
 
@@ -283,18 +284,6 @@ struct TileAssembly {
                 sw = win;
                 sr = rcp;
             }
-            // the pivot lane's two rows' columns C+2.. to LDS, the pivot row
-            // read back below (the slot is chosen by address, not by a select)
-            double* pr = lds + off_prow + (ws ? 16 : 0);
-            if constexpr (C + 2 < 16) {
-                if (lane == wl) {
-                    double* q0 = lds + off_prow;
-#pragma unroll
-                    for (int J = C + 2; J < 16; ++J) q0[J] = p0[J];
-#pragma unroll
-                    for (int J = C + 2; J < 16; ++J) q0[16 + J] = p1[J];
-                }
-            }
             if constexpr (C + 1 < 16) {
                 const double u = bcast(ws ? p1[C + 1] : p0[C + 1], wl);
                 p0[C + 1] = fma(-l0, u, p0[C + 1]);
@@ -303,15 +292,11 @@ struct TileAssembly {
             if constexpr (C + 1 < KQ) {
                 double nrcp;
                 const int nwin = pivot_of(p0[C + 1], p1[C + 1], act0, act1, k + 1, info, nrcp);
-                wave_lds_sync();
-                panel_update_lds<C + 2>(p0, p1, l0, l1, pr);
-                wave_lds_sync();  // reads done before the next step's row is written
+                panel_update<C + 2>(p0, p1, l0, l1, ws, wl);
                 panel_step<C + 1, KQ, STORE_L>(p0, p1, act0, act1, k0, lb, lds, info, lane, nwin,
                                                nrcp, sw, sr);
             } else {
-                wave_lds_sync();
-                panel_update_lds<C + 2>(p0, p1, l0, l1, pr);
-                wave_lds_sync();
+                panel_update<C + 2>(p0, p1, l0, l1, ws, wl);
             }
         }
     }
