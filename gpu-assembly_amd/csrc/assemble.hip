@@ -23,9 +23,10 @@ using namespace rbf_operators;
 using namespace rbf_amd_internal;
 
 // WPB waves per workgroup, one stencil per wave, each wave with its own
-// LDS region.
-template <class E, int WPB>
-__global__ void __launch_bounds__(64 * WPB)
+// LDS region. OCC: waves per SIMD the register allocator must leave room
+// for (1: no constraint).
+template <class E, int WPB, int OCC = 1>
+__global__ void __launch_bounds__(64 * WPB, OCC)
 k_assemble_fp64(int nstencils, const double* __restrict__ x, const double* __restrict__ y,
                 const double* __restrict__ z, const int* __restrict__ ja,
                 detail::ValueOut<double, E::NC - E::NT> out, const double* __restrict__ cx,
@@ -42,20 +43,20 @@ k_assemble_fp64(int nstencils, const double* __restrict__ x, const double* __res
     if (status && detail::lane_id() == 0) status[s] = 0;
 }
 
-template <class E, int WPB>
+template <class E, int WPB, int OCC = 1>
 hipError_t launch_fp64(const LaunchArgs& a) {
     constexpr int R = E::NC - E::NT;
     detail::ValueOut<double, R> out;
     for (int c = 0; c < R; ++c) out.p[c] = a.w[c];
     const int nblocks = (a.nstencils + WPB - 1) / WPB;
-    hipLaunchKernelGGL((k_assemble_fp64<E, WPB>), dim3(xcd_grid(nblocks)), dim3(64 * WPB), 0,
+    hipLaunchKernelGGL((k_assemble_fp64<E, WPB, OCC>), dim3(xcd_grid(nblocks)), dim3(64 * WPB), 0,
                        a.stream, a.nstencils, a.x, a.y, a.z, a.ja, out, a.cx, a.cy, a.cz, a.info,
                        a.status);
     return hipGetLastError();
 }
 
-template <class E, int WPB>
-__global__ void __launch_bounds__(64 * WPB)
+template <class E, int WPB, int OCC = 1>
+__global__ void __launch_bounds__(64 * WPB, OCC)
 k_assemble_mixed(int nstencils, const double* __restrict__ x, const double* __restrict__ y,
                  const double* __restrict__ z, const int* __restrict__ ja,
                  detail::ValueOut<double, E::R> out, int* __restrict__ info,
@@ -70,12 +71,12 @@ k_assemble_mixed(int nstencils, const double* __restrict__ x, const double* __re
            status, max_refine);
 }
 
-template <class E, int WPB>
+template <class E, int WPB, int OCC = 1>
 hipError_t launch_mixed(const LaunchArgs& a) {
     detail::ValueOut<double, E::R> out;
     for (int c = 0; c < E::R; ++c) out.p[c] = a.w[c];
     const int nblocks = (a.nstencils + WPB - 1) / WPB;
-    hipLaunchKernelGGL((k_assemble_mixed<E, WPB>), dim3(xcd_grid(nblocks)), dim3(64 * WPB), 0,
+    hipLaunchKernelGGL((k_assemble_mixed<E, WPB, OCC>), dim3(xcd_grid(nblocks)), dim3(64 * WPB), 0,
                        a.stream, a.nstencils, a.x, a.y, a.z, a.ja, out, a.info, a.status,
                        a.max_refine);
     return hipGetLastError();
@@ -88,34 +89,46 @@ struct Variant {
     LaunchFn fn;
 };
 
-constexpr int kVariants = 6;
+constexpr int kVariants = 7;
 
 // The tuning space per shape (detail::LuTune): pivot-row broadcast
 // (readlane / LDS / half or a third of the columns through LDS), pivot search (shuffle / DPP with the
 // rows combined in SGPRs / all in DPP), kernel-block fill (own row / each
 // pair once), back-substitution broadcast (readlane / LDS), and whether each
-// step's zeroed column is materialised at its step. Variant 0, the default
-// before autotuning, is the fastest measured on the n = 35 workload.
-template <template <class, int> class L, template <class> class Eng>
+// step's zeroed column is materialised at its step; and the waves per SIMD
+// the registers are sized for. Variant 0, the default before autotuning, is
+// the fastest measured on the n = 35 workload. In FP64 that is four waves
+// per SIMD (128 registers, 44 B/lane in scratch; 13.70 against 14.40 ms at
+// three waves with 140 registers). The mixed form's refinement does not fit
+// 128 registers (20.3 against 16.3 ms), so its list keeps that variant
+// second (MIXED).
+template <template <class, int, int> class L, template <class> class Eng, bool MIXED = false>
 struct VariantSet {
+    static constexpr Variant occ4 = {"bc=third pv=dpp-bcast pin fill=pairs bs=lds occ=4",
+                                     &L<Eng<detail::LuTune<4, 2, 1, 1, 1>>, 1, 4>::run};
     static constexpr Variant list[kVariants] = {
-        {"bc=third pv=dpp-bcast pin fill=pairs", &L<Eng<detail::LuTune<4, 2, 1, 0, 1>>, 1>::run},
-        {"bc=half pv=dpp-bcast pin", &L<Eng<detail::LuTune<2, 2, 0, 0, 1>>, 1>::run},
-        {"bc=half pv=dpp", &L<Eng<detail::LuTune<2, 1>>, 1>::run},
-        {"bc=readlane pv=shfl", &L<Eng<detail::LuTune<0, 0>>, 1>::run},
-        {"bc=half pv=dpp-bcast pin fill=pairs", &L<Eng<detail::LuTune<2, 2, 1, 0, 1>>, 1>::run},
+        MIXED ? Variant{"bc=third pv=dpp-bcast pin fill=pairs",
+                        &L<Eng<detail::LuTune<4, 2, 1, 0, 1>>, 1, 1>::run}
+              : occ4,
+        MIXED ? occ4
+              : Variant{"bc=third pv=dpp-bcast pin fill=pairs",
+                        &L<Eng<detail::LuTune<4, 2, 1, 0, 1>>, 1, 1>::run},
+        {"bc=half pv=dpp-bcast pin", &L<Eng<detail::LuTune<2, 2, 0, 0, 1>>, 1, 1>::run},
+        {"bc=half pv=dpp", &L<Eng<detail::LuTune<2, 1>>, 1, 1>::run},
+        {"bc=readlane pv=shfl", &L<Eng<detail::LuTune<0, 0>>, 1, 1>::run},
+        {"bc=half pv=dpp-bcast pin fill=pairs", &L<Eng<detail::LuTune<2, 2, 1, 0, 1>>, 1, 1>::run},
         {"bc=readlane pv=dpp-bcast pin fill=pairs bs=lds",
-         &L<Eng<detail::LuTune<0, 2, 1, 1, 1>>, 1>::run},
+         &L<Eng<detail::LuTune<0, 2, 1, 1, 1>>, 1, 1>::run},
     };
 };
 
-template <class E, int WPB>
+template <class E, int WPB, int OCC>
 struct Fp64L {
-    static hipError_t run(const LaunchArgs& a) { return launch_fp64<E, WPB>(a); }
+    static hipError_t run(const LaunchArgs& a) { return launch_fp64<E, WPB, OCC>(a); }
 };
-template <class E, int WPB>
+template <class E, int WPB, int OCC>
 struct MixL {
-    static hipError_t run(const LaunchArgs& a) { return launch_mixed<E, WPB>(a); }
+    static hipError_t run(const LaunchArgs& a) { return launch_mixed<E, WPB, OCC>(a); }
 };
 
 struct Shape {
@@ -181,7 +194,7 @@ template <int DIM, int N, int P, int Q>
 constexpr Shape diffmix() {
     using E = detail::WaveMixed<DIM, N, P, Q>;
     return {DIM, N, P, Q, RBFA_OPS_DIFF, 0, RBFA_PREC_MIXED, E::NT, DIM + 1,
-            VariantSet<MixL, MixEng<DIM, N, P, Q>::template E>::list, kVariants};
+            VariantSet<MixL, MixEng<DIM, N, P, Q>::template E, true>::list, kVariants};
 }
 
 // The shapes this library ships: the stencil families of the workloads it

@@ -223,6 +223,22 @@ __device__ __forceinline__ float bcast(float v, int l) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 
+// A value equal in every lane, moved to scalar registers: it then costs no
+// vector register while it waits (the stencil's index, CSR base and
+// radius live from the gather to the scatter).
+__device__ __forceinline__ double wave_uniform(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readfirstlane(static_cast<int>(b));
+    const int hi = __builtin_amdgcn_readfirstlane(static_cast<int>(b >> 32));
+    return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+__device__ __forceinline__ size_t wave_uniform(size_t v) {
+    const unsigned lo = static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(v)));
+    const unsigned hi =
+        static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(v >> 32)));
+    return (static_cast<size_t>(hi) << 32) | lo;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_max(T v) {
 #pragma unroll
@@ -469,7 +485,12 @@ __device__ __forceinline__ void elim_from(T (&row)[NC], ElimState<NT, T>& st, T*
         else
             asm volatile("" : "+v"(st.myrcp), "+v"(st.mystep));
         st.perm[K] = p;
-        update_cols<K + 1, Tune::BC>(row, l, p, pb, is_piv);
+        // The first steps update almost every column: their pivot rows go
+        // lane-to-SGPR only, with no LDS-read temporaries at the register
+        // peak (n = 35 at four waves per SIMD: 36 instead of 84 B/lane in
+        // scratch, 13.72 instead of 13.94 ms).
+        constexpr int BCK = K < 5 ? 0 : Tune::BC;
+        update_cols<K + 1, BCK>(row, l, p, pb, is_piv);
         elim_from<K + 1, NT, NC, Tune, STORE_L>(row, st, pb, lane);
     }
 }
@@ -561,7 +582,7 @@ struct WaveAssembly {
         const double pz = DIM == 3 ? z[jn] - z[s] : 0.0;
         double d2 = px * px + py * py;
         if constexpr (DIM == 3) d2 = d2 + pz * pz;
-        double rho2 = wave_max(lane < N ? d2 : 0.0);
+        double rho2 = wave_uniform(wave_max(lane < N ? d2 : 0.0));
         if (!(rho2 > 0.0)) rho2 = 1.0;
         const double rho = sqrt(rho2);
         rho2_out = rho2;
@@ -758,7 +779,7 @@ struct WaveAssembly {
 
     // Stencil s, whose CSR row starts at base (s*N for a fixed-width graph).
     template <class Out>
-    __device__ __forceinline__ static void run(const int s, const size_t base,
+    __device__ __forceinline__ static void run(const int s_arg, const size_t base_arg,
                                const double* __restrict__ x,
                                const double* __restrict__ y, const double* __restrict__ z,
                                const int* __restrict__ ja, Out out,
@@ -766,6 +787,8 @@ struct WaveAssembly {
                                const double* __restrict__ cz, double* lds,
                                int* __restrict__ info) {
 #pragma clang fp contract(off)
+        const int s = __builtin_amdgcn_readfirstlane(s_arg);
+        const size_t base = wave_uniform(base_arg);
         const int* ja_row = ja + base;
         double row[NC];
         double rho2;
