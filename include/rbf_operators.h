@@ -285,6 +285,12 @@ __device__ __forceinline__ unsigned long long wave_ballot(bool pred) {
     return __builtin_amdgcn_ballot_w64(pred);
 }
 
+// A wave-uniform 64-bit lane mask as this lane's predicate: the mask itself
+// becomes the condition register, no vector instruction.
+__device__ __forceinline__ bool lane_in(unsigned long long m) {
+    return __builtin_amdgcn_inverse_ballot_w64(m);
+}
+
 // Lowest lane, among the candidates, of largest |a| (NaN never wins):
 // the same choice as a serial "first strict maximum" scan in lane order.
 // Returns -1 when there is no candidate. PV selects the reduction:
@@ -335,6 +341,41 @@ __device__ __forceinline__ int pivot_lane(float a, bool cand) {
         const unsigned key = cand ? __float_as_uint(fabsf(a)) : 0u;
         const unsigned mk = pivot_umax<PV>(key);
         const unsigned long long hit = pivot_ballot<PV>(cand && key == mk);
+        return hit ? __ffsll(hit) - 1 : -1;
+    }
+}
+
+// The same choice from a wave-uniform mask of the candidate rows. Every
+// ballot is of a plain compare and the masks are combined in scalar
+// registers (a ballot of a combined predicate costs a select and a compare
+// on the vector unit).
+template <int PV>
+__device__ __forceinline__ int pivot_lane_mask(double a, unsigned long long act) {
+    if constexpr (PV == 0) {
+        return pivot_lane<0>(a, lane_in(act));
+    } else {
+        const unsigned long long cm = act & wave_ballot(a == a);  // not NaN
+        const double fa = fabs(a);
+        const unsigned hi = lane_in(cm) ? static_cast<unsigned>(__double2hiint(fa)) : 0u;
+        const unsigned mh = pivot_umax<PV>(hi);
+        unsigned long long hit = cm & wave_ballot(hi == mh);
+        if (__popcll(hit) > 1) {
+            const unsigned lo = lane_in(hit) ? static_cast<unsigned>(__double2loint(fa)) : 0u;
+            const unsigned ml = pivot_umax<PV>(lo);
+            hit &= wave_ballot(lo == ml);
+        }
+        return hit ? __ffsll(hit) - 1 : -1;
+    }
+}
+template <int PV>
+__device__ __forceinline__ int pivot_lane_mask(float a, unsigned long long act) {
+    if constexpr (PV == 0) {
+        return pivot_lane<0>(a, lane_in(act));
+    } else {
+        const unsigned long long cm = act & wave_ballot(a == a);
+        const unsigned key = lane_in(cm) ? __float_as_uint(fabsf(a)) : 0u;
+        const unsigned mk = pivot_umax<PV>(key);
+        const unsigned long long hit = cm & wave_ballot(key == mk);
         return hit ? __ffsll(hit) - 1 : -1;
     }
 }
@@ -439,7 +480,8 @@ __device__ __forceinline__ void update_cols(T (&row)[NC], T l, int p, T* pb, boo
 // State of the row-per-lane elimination.
 template <int NT, typename T>
 struct ElimState {
-    bool done = false;  // this lane's row has been a pivot row
+    bool done = false;  // this lane's row has been a pivot row (workgroup engines)
+    unsigned long long done_mask = 0;  // lanes whose row has been a pivot row (wave engines)
     int mystep = 1 << 20;  // step at which it was (its unknown's index); never, for padding lanes
     T myrcp = T(0);     // 1 / its pivot
     int info = 0;       // first zero (or NaN-only) pivot, 1-based
@@ -453,30 +495,36 @@ struct ElimState {
 template <int K, int NT, int NC, class Tune, bool STORE_L, typename T>
 __device__ __forceinline__ void elim_from(T (&row)[NC], ElimState<NT, T>& st, T* pb, int lane) {
     if constexpr (K < NT) {
-        const bool active = lane < NT && !st.done;
+        // rows still active, the pivot lane and the rows eliminated as
+        // wave-uniform masks: the per-lane selects read them directly
+        constexpr unsigned long long rows = NT >= 64 ? ~0ull : ((1ull << NT) - 1);
+        const unsigned long long act = rows & ~st.done_mask;
         const T a = row[K];
-        int p = pivot_lane<Tune::PV>(a, active);
+        int p = pivot_lane_mask<Tune::PV>(a, act);
         if (p < 0) {  // every candidate NaN
-            p = __ffsll(__ballot(active)) - 1;
+            p = __ffsll(act) - 1;
             if (!st.info) st.info = K + 1;
         }
         p = __builtin_amdgcn_readfirstlane(p);
         const T piv = bcast(a, p);
         if (piv == T(0) && !st.info) st.info = K + 1;
+        // settle info at this step (deferred, it keeps every step's pivot
+        // and candidate flag in scalar registers, which then spill)
+        st.info = __builtin_amdgcn_readfirstlane(st.info);
+        asm volatile("" : "+s"(st.info));
         const T rcp = T(1) / piv;
-        const bool is_piv = lane == p;
-        const bool elim = active && !is_piv;
+        const unsigned long long pm = 1ull << p;
+        const bool is_piv = lane_in(pm);
+        const bool elim = lane_in(act & ~pm);
         const T l = elim ? a * rcp : T(0);
         if constexpr (STORE_L) {
             row[K] = elim ? l : a;
         } else {
-            row[K] = active ? T(0) : a;
+            row[K] = lane_in(act) ? T(0) : a;
         }
-        if (is_piv) {
-            st.done = true;
-            st.mystep = K;
-            st.myrcp = rcp;
-        }
+        st.mystep = is_piv ? K : st.mystep;
+        st.myrcp = is_piv ? rcp : st.myrcp;
+        st.done_mask |= pm;
         // Materialise this step's selects now: left alone, the compiler sinks
         // all NT of them to the end and keeps NT reciprocals and NT lane
         // masks live meanwhile (the masks spilled to VGPR lanes).
@@ -742,13 +790,15 @@ struct WaveAssembly {
         elim_from<0, NT, NC, Tune, false>(row, st, lds, lane);
         // back substitution: row[k] holds u_ik for rows pivotal before step
         // k and 0 for every other row, so all lanes update alike. The
-        // pivot lane of step k passes t_c = b_c / u_kk (as b_c * rcp) to
-        // every lane, through SGPRs or (Tune::BS) through LDS in two slots
-        // taken in turn.
+        // pivot lane of step k (the lane whose mystep is k, found by a
+        // ballot rather than kept in NT scalar registers through the
+        // elimination) passes t_c = b_c / u_kk (as b_c * rcp) to every
+        // lane, through SGPRs or (Tune::BS) through LDS in two slots taken
+        // in turn.
         if constexpr (Tune::BS == 0) {
 #pragma unroll
             for (int k = NT - 1; k >= 0; --k) {
-                const int p = st.perm[k];
+                const int p = __ffsll(wave_ballot(st.mystep == k)) - 1;
 #pragma unroll
                 for (int c = 0; c < R; ++c) {
                     const double t = bcast(row[NT + c] * st.myrcp, p);
@@ -761,7 +811,7 @@ struct WaveAssembly {
 #pragma unroll
             for (int k = NT - 1; k >= 0; --k) {
                 double* tb = lds + (k & 1) * TS;
-                if (lane == st.perm[k]) {
+                if (st.mystep == k) {
 #pragma unroll
                     for (int c = 0; c < R; ++c) tb[c] = row[NT + c] * st.myrcp;
                 }
