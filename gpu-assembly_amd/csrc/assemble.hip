@@ -98,14 +98,15 @@ constexpr int kVariants = 7;
 // step's zeroed column is materialised at its step; and the waves per SIMD
 // the registers are sized for. Variant 0, the default before autotuning, is
 // the fastest measured on the n = 35 workload. In FP64 that is four waves
-// per SIMD (128 registers, 44 B/lane in scratch; 13.70 against 14.40 ms at
-// three waves with 140 registers). The mixed form's refinement does not fit
-// 128 registers (20.3 against 16.3 ms), so its list keeps that variant
-// second (MIXED).
+// per SIMD: 128 registers (44 B/lane in scratch) and under 10 KB of LDS per
+// wave, so 16 waves fit a CU (13.14 ms in a probe, against 13.54 ms with 15
+// waves and 14.3 ms at three waves per SIMD). The mixed form's refinement
+// does not fit 128 registers (18.9 against 16.2 ms), so its list keeps that
+// variant second (MIXED).
 template <template <class, int, int> class L, template <class> class Eng, bool MIXED = false>
 struct VariantSet {
-    static constexpr Variant occ4 = {"bc=third pv=dpp-bcast pin fill=pairs bs=lds occ=4",
-                                     &L<Eng<detail::LuTune<4, 2, 1, 1, 1>>, 1, 4>::run};
+    static constexpr Variant occ4 = {"bc=third pv=dpp-bcast pin fill=pairs,y-by-lane bs=lds occ=4",
+                                     &L<Eng<detail::LuTune<4, 2, 2, 1, 1>>, 1, 4>::run};
     static constexpr Variant list[kVariants] = {
         MIXED ? Variant{"bc=third pv=dpp-bcast pin fill=pairs",
                         &L<Eng<detail::LuTune<4, 2, 1, 0, 1>>, 1, 1>::run}

@@ -549,7 +549,8 @@ __device__ __forceinline__ void elim_from(T (&row)[NC], ElimState<NT, T>& st, T*
 //       thirds through LDS, 4 = one third through LDS
 //   PV  pivot search: see pivot_lane
 //   FL  kernel-block fill: 0 = every lane evaluates its own row (the
-//       other point through SGPRs), 1 = each pair once, through LDS
+//       other point through SGPRs), 1 = each pair once, through LDS,
+//       2 = as 1 with y read from the node's lane (less LDS per wave)
 //   BS  back substitution: 0 = t through SGPRs (v_readlane), 1 = LDS
 //   PN  1 = the zeroed column of each step materialised at its step
 template <int BC_, int PV_, int FL_ = 0, int BS_ = 0, int PN_ = 0>
@@ -600,10 +601,14 @@ struct WaveAssembly {
     static constexpr int lds_fill = M * N + (INTERP ? DIM * R + R * M : 0);
     static constexpr int lds_shared = cmax(cmax(lds_fill, N * N), lds_pivot);
     static constexpr bool pair_fill = Tune::FL != 0;
+    // FL = 2: the pair fill reads y from the node's lane (cross-lane
+    // shuffle) instead of LDS, which trims the wave's LDS below 10 KB
+    static constexpr bool y_by_lane = Tune::FL == 2;
     static constexpr int off_pts = pair_fill ? (lds_shared + 1) / 2 * 2 : lds_pivot;
     static constexpr int off_poly = pair_fill ? 0 : lds_pivot + DIM * N;
     static constexpr int lds_doubles =
-        pair_fill ? (off_pts + DIM * N + 1) / 2 * 2 : (off_poly + lds_fill + 1) / 2 * 2;
+        pair_fill ? (off_pts + (y_by_lane ? DIM - 1 : DIM) * N + 1) / 2 * 2
+                  : (off_poly + lds_fill + 1) / 2 * 2;
     // kernel-block entries above the diagonal, and how many per lane
     static constexpr int npairs = N * (N - 1) / 2;
     static constexpr int pair_slots = (npairs + 63) / 64;
@@ -641,8 +646,8 @@ struct WaveAssembly {
         basis(xs, ys, zs, mono, 1);
         if (lane < N) {
             pts[lane] = xs;
-            pts[N + lane] = ys;
-            if constexpr (DIM == 3) pts[2 * N + lane] = zs;
+            if constexpr (!y_by_lane) pts[N + lane] = ys;
+            if constexpr (DIM == 3) pts[(y_by_lane ? 1 : 2) * N + lane] = zs;
 #pragma unroll
             for (int t = 0; t < M; ++t) polyT[t * N + lane] = mono[t];
         }
@@ -737,7 +742,7 @@ struct WaveAssembly {
             for (int c = 0; c < NC; ++c) row[c] = T(0);
         }
         wave_lds_sync();  // monomials and centres read: the kernel block overwrites them
-        if constexpr (pair_fill) fill_pairs(lane, pts, phiT, row);
+        if constexpr (pair_fill) fill_pairs(lane, ys, pts, phiT, row);
     }
 
     // Kernel block, each entry above the diagonal evaluated once: pair
@@ -747,22 +752,34 @@ struct WaveAssembly {
     // in floating point, so both entries get the value a direct fill
     // computes.
     template <typename T>
-    __device__ __forceinline__ static void fill_pairs(int lane, const double* pts, double* phiT,
-                                                      T (&row)[NC]) {
+    __device__ __forceinline__ static void fill_pairs(int lane, double ys, const double* pts,
+                                                      double* phiT, T (&row)[NC]) {
 #pragma clang fp contract(off)
         constexpr PHS<Q> phi{};
+        constexpr int oz = y_by_lane ? N : 2 * N;  // z in LDS (3-D)
 #pragma unroll
         for (int sl = 0; sl < pair_slots; ++sl) {
-            const int q = sl * 64 + lane;
-            if (q < npairs) {
-                const int d = q / N + 1;
-                const int i = q - (d - 1) * N;
-                const int j = i + d < N ? i + d : i + d - N;
+            // lanes past the last pair take pair 0 and write nothing (every
+            // lane joins the shuffles)
+            const int q0 = sl * 64 + lane;
+            const int q = q0 < npairs ? q0 : 0;
+            const int d = q / N + 1;
+            const int i = q - (d - 1) * N;
+            const int j = i + d < N ? i + d : i + d - N;
+            double yi, yj;
+            if constexpr (y_by_lane) {
+                yj = __shfl(ys, j, 64);
+                yi = __shfl(ys, i, 64);
+            } else {
+                yj = pts[N + j];
+                yi = pts[N + i];
+            }
+            if (q0 < npairs) {
                 const double dx = pts[j] - pts[i];
-                const double dy = pts[N + j] - pts[N + i];
+                const double dy = yj - yi;
                 double r2 = dx * dx + dy * dy;
                 if constexpr (DIM == 3) {
-                    const double dz = pts[2 * N + j] - pts[2 * N + i];
+                    const double dz = pts[oz + j] - pts[oz + i];
                     r2 = r2 + dz * dz;
                 }
                 const double v = phi(r2);
