@@ -884,7 +884,16 @@ struct WaveAssembly {
 // 187-290). Launch contract:
 //   gridDim.x = nstencils, blockDim = block_dim (one wave64),
 //   dynamic LDS = shared_memory_size() (no opt-in needed on CDNA).
+// The kernels compile with the wave LU tuning that the run-time plans run
+// before autotuning (tune, waves_per_simd): pivot row a third through LDS,
+// DPP pivot search, pairwise fill with y read from the node's lane, LDS back
+// substitution, registers sized for four waves per SIMD.
 // ---------------------------------------------------------------------------
+
+namespace detail {
+using DropInTune = LuTune<4, 2, 2, 1, 1>;
+constexpr int kDropInWavesPerSimd = 4;
+}  // namespace detail
 
 template <int N_, int NRHS_, int P_, int Q_ = 3, unsigned int ARCH_ = 950, typename T_ = double>
 struct interp_config {
@@ -895,7 +904,9 @@ struct interp_config {
     static constexpr int nt = N + npoly;
     static constexpr int dim = 2;
     static_assert(std::is_same_v<value_type, double>, "the wave LU path computes in FP64");
-    using engine = detail::WaveAssembly<2, N, P, Q, NRHS, true>;
+    using tune = detail::DropInTune;
+    static constexpr int waves_per_simd = detail::kDropInWavesPerSimd;
+    using engine = detail::WaveAssembly<2, N, P, Q, NRHS, true, tune>;
     static constexpr auto& arch = gfx_arch::limits_v<ARCH>;
     static constexpr std::size_t shared_memory_size() {
         return sizeof(double) * static_cast<std::size_t>(engine::lds_doubles);
@@ -916,7 +927,9 @@ struct laplace_config {
     static constexpr int npoly = npoly_dim(DIM_, P_);
     static constexpr int nt = N + npoly;
     static_assert(std::is_same_v<value_type, double>, "the wave LU path computes in FP64");
-    using engine = detail::WaveAssembly<DIM_, N, P, Q, NRHS, false>;
+    using tune = detail::DropInTune;
+    static constexpr int waves_per_simd = detail::kDropInWavesPerSimd;
+    using engine = detail::WaveAssembly<DIM_, N, P, Q, NRHS, false, tune>;
     static constexpr auto& arch = gfx_arch::limits_v<ARCH>;
     static constexpr std::size_t shared_memory_size() {
         return sizeof(double) * static_cast<std::size_t>(engine::lds_doubles);
@@ -931,14 +944,15 @@ struct laplace_config {
 // device-readable table of NRHS value arrays laid out like ja; xc, yc are
 // the NRHS centres in the stencil-local (shifted, unscaled) frame.
 template <class Config, typename T = typename Config::value_type>
-__global__ void __launch_bounds__(64) assemble_interp_weights(const int nstencils,
-                                                              const T* __restrict__ x,
-                                                              const T* __restrict__ y,
-                                                              const int* __restrict__ ja,
-                                                              T* const* __restrict__ A,
-                                                              const T* __restrict__ xc,
-                                                              const T* __restrict__ yc,
-                                                              int* __restrict__ info) {
+__global__ void __launch_bounds__(64, Config::waves_per_simd)
+assemble_interp_weights(const int nstencils,
+                        const T* __restrict__ x,
+                        const T* __restrict__ y,
+                        const int* __restrict__ ja,
+                        T* const* __restrict__ A,
+                        const T* __restrict__ xc,
+                        const T* __restrict__ yc,
+                        int* __restrict__ info) {
     extern __shared__ double rbf_lds_interp[];
     const int s = blockIdx.x;
     if (s >= nstencils) return;
@@ -949,13 +963,14 @@ __global__ void __launch_bounds__(64) assemble_interp_weights(const int nstencil
 // Laplacian / d/dx / d/dy weights at every node: A[0] Laplacian, A[1]
 // d/dx, A[2] d/dy (A[3] d/dz in 3-D, with z non-null).
 template <class Config, typename T = typename Config::value_type>
-__global__ void __launch_bounds__(64) assemble_laplace_weights(const int nstencils,
-                                                               const T* __restrict__ x,
-                                                               const T* __restrict__ y,
-                                                               const int* __restrict__ ja,
-                                                               T* const* __restrict__ A,
-                                                               int* __restrict__ info,
-                                                               const T* __restrict__ z = nullptr) {
+__global__ void __launch_bounds__(64, Config::waves_per_simd)
+assemble_laplace_weights(const int nstencils,
+                         const T* __restrict__ x,
+                         const T* __restrict__ y,
+                         const int* __restrict__ ja,
+                         T* const* __restrict__ A,
+                         int* __restrict__ info,
+                         const T* __restrict__ z = nullptr) {
     extern __shared__ double rbf_lds_laplace[];
     const int s = blockIdx.x;
     if (s >= nstencils) return;
