@@ -96,6 +96,26 @@ def test_assemble_rejects_null_plan():
                                  None, None) == R.ERR_ARG
 
 
+@pytest.mark.gpu
+def test_assemble_argument_checks_before_any_launch():
+    """Empty batch: OK with nothing launched (the reference's kernel over
+    zero blocks is a no-op); negative counts, missing arrays and CSR sizes
+    past int32 are errors, returned before any launch. (Creating the plan
+    needs the device.)"""
+    L = R.lib()
+    plan = R.Plan(2, 35, 3, 5, "diff")
+    h = plan._h
+    nul = [None] * 10
+    assert L.rbfa_assemble(h, 0, *nul, None) == R.OK
+    assert L.rbfa_assemble(h, -1, *nul, None) == R.ERR_ARG
+    assert L.rbfa_assemble(h, 10, *nul, None) == R.ERR_ARG
+    dummy = ctypes.c_void_p(8)
+    tab = (ctypes.c_void_p * 3)(8, 8, 8)
+    big = 2**31 // 35 + 1   # nstencils * n past INT32_MAX
+    assert L.rbfa_assemble(h, big, dummy, dummy, None, dummy, tab, None, None, None, None,
+                           None, None) == R.ERR_ARG
+
+
 @pytest.mark.skipif(not (shutil.which("flang") or shutil.which("amdflang")),
                     reason="no Fortran compiler")
 def test_fortran_bind_c_reaches_the_library(tmp_path):
