@@ -453,7 +453,8 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the assembly "
-              "shards by node range with no exchange; see DESIGN.md)", file=sys.stderr)
+              "would shard by node range, each GPU assembling the rows of its own nodes "
+              "with no exchange)", file=sys.stderr)
         sys.exit(2)
 
     import torch
