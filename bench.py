@@ -71,9 +71,14 @@ def rel_errors(w_gpu, w_ref, rows):
     return np.stack(out, 1)
 
 
-def test_function(name, pts):
-    """f and its exact Laplacian and gradient at the nodes."""
+def test_function(name, pts, centres=None):
+    """f and its exact Laplacian and gradient at the nodes (for an
+    interpolation set: f and its values at each node + centre c)."""
     x, y = pts[0], pts[1]
+    if name == "sincos_interp":
+        def f2(a, b):
+            return np.sin(2 * np.pi * a) * np.cos(2 * np.pi * b)
+        return f2(x, y), [f2(x + cx, y + cy) for cx, cy in zip(*centres)]
     if name == "sincos":
         f = np.sin(2 * np.pi * x) * np.cos(2 * np.pi * y)
         return f, [-8 * np.pi ** 2 * f,
@@ -89,10 +94,10 @@ def test_function(name, pts):
     raise ValueError(name)
 
 
-def consistency(w, ja, rows, pts, fname, inner):
+def consistency(w, ja, rows, pts, fname, inner, centres=None):
     """Max relative error of the assembled operators applied to f, on the
     nodes selected by `inner` (away from the boundary)."""
-    f, exact = test_function(fname, pts)
+    f, exact = test_function(fname, pts, centres)
     fj = f[ja]
     errs = []
     for wi, ex in zip(w, exact):
@@ -141,6 +146,7 @@ WORKLOADS = {
     "square_4k_n21": ("sincos", 1 << 20),
     "square_1m_n56": ("exp2", 1 << 20),
     "cube_2m_n80": ("exp3", 1 << 17),
+    "interp_1m_d2q9": ("sincos_interp", 1 << 20),
     "disk_2m_uneven": ("sincos", 1 << 22),
     "apply_1m_n35": ("sincos", 1 << 20),
     "nodes_1m_n35": ("sincos", 1 << 20),
@@ -298,6 +304,9 @@ def run_apply(args, R, workloads, torch, dev):
 def run_fixed(args, name, R, workloads, torch, dev):
     W = workloads.build(name, seed=args.seed)
     dim, n, P, Q, nst = W["dim"], W["n"], W["P"], W["Q"], W["nstencils"]
+    nrhs = workloads.INTERP_NRHS.get(name, 0)
+    if nrhs:
+        W["centres"] = workloads.d2q9_centres(1.0 / round(nst ** (1.0 / dim)))
     log(f"[{name}] host prep: {W['host_s']}")
     pts = [W["x"], W["y"]] + ([W["z"]] if dim == 3 else [])
     dpts = [torch.from_numpy(a).to(dev) for a in pts]
@@ -309,22 +318,25 @@ def run_fixed(args, name, R, workloads, torch, dev):
     res = {}
     for form in forms:
         try:
-            plan = R.Plan(dim, n, P, Q, "diff", precision=form, max_refine=3)
+            plan = R.Plan(dim, n, P, Q, "interp" if nrhs else "diff", nrhs=nrhs, precision=form,
+                          max_refine=3)
         except R.RbfError as e:
             log(f"[{form}] {e}")
             continue
         w = [torch.empty(nnz, dtype=torch.float64, device=dev) for _ in range(plan.nops)]
         z = dpts[2] if dim == 3 else None
+        cen = (tuple(torch.from_numpy(c).to(dev) for c in W["centres"]) if nrhs else None)
 
         def step():
-            plan.assemble(nst, dpts[0], dpts[1], ja, w, z=z, info=info, status=status)
+            plan.assemble(nst, dpts[0], dpts[1], ja, w, z=z, centres=cen, info=info,
+                          status=status)
 
         tune_ms = None
         if args.variant is not None:
             plan.variant = args.variant
         elif not args.no_tune and plan.num_variants > 1:
-            tune_ms = plan.autotune(nst, dpts[0], dpts[1], ja, w, z=z, info=info, status=status,
-                                    reps=2)
+            tune_ms = plan.autotune(nst, dpts[0], dpts[1], ja, w, z=z, centres=cen, info=info,
+                                    status=status, reps=2)
             log(f"[{form}] autotune: " + ", ".join(
                 f"{plan.variant_name(v)}={t:.3f}ms" for v, t in enumerate(tune_ms)))
         log(f"[{form}] variant {plan.variant_name()}; warmup {args.warmup}, "
@@ -354,8 +366,9 @@ def oracle_fixed(W, res, n, dim, sample):
     s0 = (nst - cnt) // 2
     log(f"CPU oracle: {cnt} stencils on {cores} threads")
     t0 = time.perf_counter()
+    op = {"op_set": 1, "centres": W["centres"]} if "centres" in W else {}
     wref, _ = O.assemble(dim, n, W["P"], W["Q"], W["x"], W["y"], W["ja"], z=W["z"], s0=s0,
-                         count=cnt, nthreads=cores)
+                         count=cnt, nthreads=cores, **op)
     cpu_s = time.perf_counter() - t0
     cpu = {"value": cnt / cpu_s, "unit": "stencils/s", "cores": cores, "kind": "port",
            "sample": f"{cnt} contiguous stencils (from {s0}) of the same workload, FP64 oracle "
@@ -364,7 +377,7 @@ def oracle_fixed(W, res, n, dim, sample):
     c1 = max(1, cnt // 16)
     t0 = time.perf_counter()
     O.assemble(dim, n, W["P"], W["Q"], W["x"], W["y"], W["ja"], z=W["z"], s0=s0, count=c1,
-               nthreads=1)
+               nthreads=1, **op)
     one_s = time.perf_counter() - t0
     cpu["single_core"] = {"value": c1 / one_s, "unit": "stencils/s", "cores": 1,
                           "sample": f"{c1} contiguous stencils (from {s0}), {one_s:.2f} s"}
@@ -554,7 +567,8 @@ def main():
         if d.get("class_counts") is not None:
             extra[form]["class_counts"] = d["class_counts"]
         extra[form]["consistency_fn"] = fname
-        extra[form]["consistency_rel_err"] = consistency(d["w"], W["ja"], rows, pts, fname, inner)
+        extra[form]["consistency_rel_err"] = consistency(d["w"], W["ja"], rows, pts, fname, inner,
+                                                         W.get("centres"))
     if "fp64" in res and "mixed" in res:
         extra["mixed_over_fp64"] = res["fp64"]["ms"] / res["mixed"]["ms"]
 
@@ -597,7 +611,7 @@ def main():
         "form": head,
         "data": "synthetic (seeded node generators, curve order, host kNN)",
         "config": {"workload": name, "nodes": nst, "n": n_desc, "dim": dim,
-                   "ops": dim + 1, "form": head},
+                   "ops": workloads.INTERP_NRHS.get(name) or dim + 1, "form": head},
         "roofline": {"bound": "mfma", "peak_kind": "fp64 (vector = matrix on MI355X)",
                      "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS,

@@ -17,7 +17,21 @@ SHAPES = {
     "square_1m_n35": (2, 1024, 35, 3, 5, "hilbert"),
     "square_1m_n56": (2, 1024, 56, 4, 7, "hilbert"),
     "cube_2m_n80": (3, 128, 80, 3, 3, "morton"),
+    "interp_1m_d2q9": (2, 1024, 9, 2, 3, "hilbert"),
 }
+
+# Interpolation workloads: number of centres. D2Q9 semi-Lagrangian streaming
+# (the reference's CUDA-Fortran demo, main.f90:16,55: n = 9, P = 2, NRHS = 9,
+# centres half-way along each of the nine lattice directions).
+INTERP_NRHS = {"interp_1m_d2q9": 9}
+
+
+def d2q9_centres(h):
+    """The nine interpolation centres in the stencil-local frame: 0.5 h c_i
+    for the D2Q9 lattice velocities c_i (rest, 4 axis, 4 diagonal)."""
+    cx = 0.5 * h * np.array([0, 1, 0, -1, 0, 1, -1, -1, 1], float)
+    cy = 0.5 * h * np.array([0, 0, 1, 0, -1, 1, 1, -1, -1], float)
+    return cx, cy
 
 
 def build(name=None, dim=2, m=64, n=21, P=2, Q=3, order=None, seed=1, jitter=0.25):
