@@ -89,7 +89,7 @@ struct Variant {
     LaunchFn fn;
 };
 
-constexpr int kVariants = 7;
+constexpr int kVariants = 8;
 
 // The tuning space per shape (detail::LuTune): pivot-row broadcast
 // (readlane / LDS / half or a third of the columns through LDS), pivot search (shuffle / DPP with the
@@ -120,6 +120,8 @@ struct VariantSet {
         {"bc=half pv=dpp-bcast pin fill=pairs", &L<Eng<detail::LuTune<2, 2, 1, 0, 1>>, 1, 1>::run},
         {"bc=readlane pv=dpp-bcast pin fill=pairs bs=lds",
          &L<Eng<detail::LuTune<0, 2, 1, 1, 1>>, 1, 1>::run},
+        {"bc=third pv=dpp-bcast pin fill=pairs,y-by-lane bs=lds rcp=scalar-guard occ=4",
+         &L<Eng<detail::LuTune<4, 2, 2, 1, 1, 1>>, 1, 4>::run},
     };
 };
 

@@ -239,6 +239,34 @@ __device__ __forceinline__ size_t wave_uniform(size_t v) {
     return (static_cast<size_t>(hi) << 32) | lo;
 }
 
+// 1 / b for a wave-uniform FP64 b, bit-identical to the IEEE division
+// 1.0 / b. hipcc expands that division as v_div_scale (operand scaling), a
+// v_rcp_f64 seed r, two Newton-Raphson steps r = r + r (1 - b r), the
+// correction q = r + r (1 - b r) (v_div_fmas) and v_div_fixup (special
+// operands). For 2^-700 <= |b| < 2^701 the scaling is the identity (the
+// exponent gap to 1.0 stays below 768, b and 1/b are normal) and the fixup
+// returns q, so the same seed and the same six FMAs in the same order give
+// the same bits, four VALU instructions fewer. The exponent test runs on
+// scalar registers; any other b (0, inf, NaN, extreme exponents) takes the
+// division itself.
+__device__ __forceinline__ double recip_uniform(double b) {
+#pragma clang fp contract(off)
+    const unsigned hi = static_cast<unsigned>(
+        __builtin_amdgcn_readfirstlane(static_cast<int>(__double_as_longlong(b) >> 32)));
+    const unsigned ex = (hi >> 20) & 0x7ffu;
+    if (ex - (1023u - 700u) <= 1400u) {
+        double r = __builtin_amdgcn_rcp(b);
+        double e = fma(-b, r, 1.0);
+        r = fma(r, e, r);
+        e = fma(-b, r, 1.0);
+        r = fma(r, e, r);
+        e = fma(-b, r, 1.0);
+        return fma(e, r, r);
+    }
+    return 1.0 / b;
+}
+__device__ __forceinline__ float recip_uniform(float b) { return 1.0f / b; }
+
 template <typename T>
 __device__ __forceinline__ T wave_max(T v) {
 #pragma unroll
@@ -512,7 +540,11 @@ __device__ __forceinline__ void elim_from(T (&row)[NC], ElimState<NT, T>& st, T*
         // and candidate flag in scalar registers, which then spill)
         st.info = __builtin_amdgcn_readfirstlane(st.info);
         asm volatile("" : "+s"(st.info));
-        const T rcp = T(1) / piv;
+        T rcp;
+        if constexpr (Tune::RX != 0)
+            rcp = recip_uniform(piv);
+        else
+            rcp = T(1) / piv;
         const unsigned long long pm = 1ull << p;
         const bool is_piv = lane_in(pm);
         const bool elim = lane_in(act & ~pm);
@@ -553,9 +585,12 @@ __device__ __forceinline__ void elim_from(T (&row)[NC], ElimState<NT, T>& st, T*
 //       2 = as 1 with y read from the node's lane (less LDS per wave)
 //   BS  back substitution: 0 = t through SGPRs (v_readlane), 1 = LDS
 //   PN  1 = the zeroed column of each step materialised at its step
-template <int BC_, int PV_, int FL_ = 0, int BS_ = 0, int PN_ = 0>
+//   RX  1 = pivot reciprocals by recip_uniform (four VALU instructions
+//       fewer per step, a scalar branch per step; pays where registers
+//       are not the limit)
+template <int BC_, int PV_, int FL_ = 0, int BS_ = 0, int PN_ = 0, int RX_ = 0>
 struct LuTune {
-    static constexpr int BC = BC_, PV = PV_, FL = FL_, BS = BS_, PN = PN_;
+    static constexpr int BC = BC_, PV = PV_, FL = FL_, BS = BS_, PN = PN_, RX = RX_;
 };
 using DefaultTune = LuTune<0, 0>;
 
