@@ -48,6 +48,10 @@ import numpy as np  # noqa: E402
 PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix alike)
 PEAK_HBM_GBS = 8000.0
 HEADLINE = "square_1m_n35"
+# Compute unit that bounds each workload's dominant kernel: the wave-LU
+# kernels issue no MFMA (FP64 VALU); the blocked-LU kernels put their
+# trailing updates on FP64 MFMA but are bound by the VALU panel chain.
+BOUND_OF = {}
 METRIC = "stencils/s assembled, 1M nodes n=35; mixed vs FP64 at <=1e-8 weight rel. err"
 
 
@@ -127,17 +131,25 @@ def hip_timed(fn, steps, warmup):
     return t0.elapsed_time(t1) / steps, wall
 
 
-def read_traffic(kernel_tag):
+def read_traffic(kernel_tag, variant=None):
     """HBM bytes per launch of the kernel, from the committed rocprofv3 --pmc
-    summary (profiles/<kernel_tag>.md, line "traffic_bytes_per_launch: N")."""
+    summary (profiles/<kernel_tag>.md, line "traffic_bytes_per_launch: N").
+    When the summary names the kernel variant it measured (line
+    "traffic_variant: NAME") and the run used another variant, the figure
+    belongs to a different kernel and None is returned."""
     path = os.path.join(ROOT, "profiles", kernel_tag + ".md")
     if not os.path.exists(path):
         return None
+    val, var = None, None
     with open(path) as f:
         for line in f:
             if line.startswith("traffic_bytes_per_launch:"):
-                return int(line.split(":")[1])
-    return None
+                val = int(line.split(":")[1])
+            elif line.startswith("traffic_variant:"):
+                var = line.split(":", 1)[1].strip()
+    if variant is not None and var is not None and var != variant:
+        return None
+    return val
 
 
 # name: (consistency test function, default CPU-oracle sample in stencils)
@@ -612,10 +624,13 @@ def main():
         "data": "synthetic (seeded node generators, curve order, host kNN)",
         "config": {"workload": name, "nodes": nst, "n": n_desc, "dim": dim,
                    "ops": workloads.INTERP_NRHS.get(name) or dim + 1, "form": head},
-        "roofline": {"bound": "mfma", "peak_kind": "fp64 (vector = matrix on MI355X)",
+        "roofline": {"bound": BOUND_OF.get((name, head), "valu"),
+                     "peak_kind": "fp64 (vector = matrix on MI355X)",
                      "achieved": achieved_tflops, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved_tflops / PEAK_FP64_TFLOPS,
-                     "flop_per_stencil": flop, "traffic": read_traffic(kernel_tag),
+                     "flop_per_stencil": flop,
+                     "traffic": read_traffic(kernel_tag, res[head]["variant"]),
+                     "traffic_profile": "profiles/%s.md" % kernel_tag,
                      "peak_measured": {k: round(v, 2) for k, v in peaks.items()},
                      "frac_of_measured_fp64_valu": achieved_tflops / peaks["fp64_valu_tflops"]},
         "cpu_baseline": cpu,
