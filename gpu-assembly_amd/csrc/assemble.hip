@@ -15,6 +15,7 @@
 #include "rbf_assembly.h"
 #include "rbf_operators.h"
 #include "rbf_mixed.h"
+#include "rbf_nullspace.h"
 #include "assemble_common.h"
 
 namespace {
@@ -152,11 +153,6 @@ struct InterpEng {
     template <class T>
     using E = detail::WaveAssembly<2, N, P, Q, R, true, T>;
 };
-template <int DIM, int N, int P, int Q>
-struct MixEng {
-    template <class T>
-    using E = detail::WaveMixed<DIM, N, P, Q, T>;
-};
 
 template <int DIM, int N, int P, int Q>
 constexpr Shape diff64() {
@@ -193,11 +189,31 @@ constexpr Shape interp64() {
     return {2, N, P, Q, RBFA_OPS_INTERP, R, RBFA_PREC_FP64, E::NT, R,
             VariantSet<Fp64L, InterpEng<N, P, Q, R>::template E>::list, kVariants};
 }
+// Mixed precision: the null-space solver (FP64 polynomial-block LU and
+// Schur complement on FP64 MFMA, FP32 LU of the reduced system, FP64
+// refinement) first, then the whole-system FP32 LU of the wave engine.
+template <int DIM, int N, int P, int Q>
+struct NsMixVariants {
+    template <class T>
+    using NS = detail::NullSpaceMixed<DIM, N, P, Q, T>;
+    template <class T>
+    using WM = detail::WaveMixed<DIM, N, P, Q, T>;
+    static constexpr Variant list[4] = {
+        {"null space: fp64 poly lu + mfma schur, fp32 lu bc=readlane occ=4",
+         &MixL<NS<detail::LuTune<0, 2, 0, 0, 1>>, 1, 4>::run},
+        {"null space: fp64 poly lu + mfma schur, fp32 lu bc=third occ=4",
+         &MixL<NS<detail::LuTune<4, 2, 0, 0, 1>>, 1, 4>::run},
+        {"null space: fp64 poly lu + mfma schur, fp32 lu bc=readlane occ=3",
+         &MixL<NS<detail::LuTune<0, 2, 0, 0, 1>>, 1, 3>::run},
+        {"whole system: fp32 lu bc=third pv=dpp-bcast pin fill=pairs",
+         &MixL<WM<detail::LuTune<4, 2, 1, 0, 1>>, 1, 1>::run},
+    };
+};
 template <int DIM, int N, int P, int Q>
 constexpr Shape diffmix() {
     using E = detail::WaveMixed<DIM, N, P, Q>;
     return {DIM, N, P, Q, RBFA_OPS_DIFF, 0, RBFA_PREC_MIXED, E::NT, DIM + 1,
-            VariantSet<MixL, MixEng<DIM, N, P, Q>::template E, true>::list, kVariants};
+            NsMixVariants<DIM, N, P, Q>::list, 4};
 }
 
 // The shapes this library ships: the stencil families of the workloads it

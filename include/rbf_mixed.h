@@ -11,7 +11,8 @@
 //      LDS), d = U^-1 L^-1 r in FP32, x += d in FP64; stop when the
 //      estimated remaining error of the weights, |d| min(1, |d|/|d_prev|),
 //      is below 1e-10 of |x| for every operator;
-//   4. a stencil that has not converged (or met a zero FP32 pivot) is
+//   4. a stencil that has not converged (or met an FP32 pivot below 1e-6
+//      of the largest matrix entry) is
 //      solved again by the FP64 path from the row kept in step 1 and
 //      flagged RBFA_STAT_FALLBACK in its status byte.
 // The polynomial block keeps FP64 where it decides anything: its entries,
@@ -87,12 +88,21 @@ struct WaveMixed {
         double* xl = lds + Base::lds_doubles;  // [R][64]
 
         float a[NC];
+        float rowmax = 0.f;
 #pragma unroll
-        for (int j = 0; j < NC; ++j) a[j] = static_cast<float>(row[j]);
+        for (int j = 0; j < NC; ++j) {
+            a[j] = static_cast<float>(row[j]);
+            if (j < NT) rowmax = fmaxf(rowmax, fabsf(a[j]));
+        }
         int perm[NT];
         int mystep;
         float myrcp;
-        const int bad = lu32(a, perm, mystep, myrcp, reinterpret_cast<float*>(lds));
+        int bad = lu32(a, perm, mystep, myrcp, reinterpret_cast<float*>(lds));
+        // a pivot below 1e-6 of the largest matrix entry (duplicated nodes,
+        // nearly singular stencils): FP32 factors that small carry no
+        // information, and refinement can appear to converge on them
+        const float smax = __uint_as_float(wave_umax(__float_as_uint(rowmax)));
+        if (wave_ballot(lane < NT && !(fabsf(myrcp) * (1e-6f * smax) < 1.f))) bad = 1;
 
         // initial solve from the eliminated right-hand sides
         float r32[R];

@@ -1,0 +1,510 @@
+// rbf_nullspace.h -- mixed-precision stencil solve on the null space of the
+// polynomial constraints (gfx950, one wave64 per stencil).
+//
+// The saddle-point system of a stencil (scaled frame, n nodes, m monomials)
+//
+//     [ A   P ] [w]   [b]        A = phi(|q_i - q_j|)    n x n
+//     [ P^T 0 ] [l] = [g]        P = monomials at q_i    n x m
+//
+// is ill-conditioned as a whole (condition numbers 2e5 at n = 35, r^5 + P3,
+// 2e7 at n = 56, r^7 + P4), but almost all of that comes from the
+// polynomial block. Restricted to the null space of P^T the kernel block is
+// well conditioned (3e3 and 4e4 for the same stencils), so FP32 factors of
+// the restriction converge under FP64 refinement in one or two steps where
+// FP32 factors of the whole system take three, or never converge.
+//
+// Per stencil:
+//   A  FP64 LU with partial pivoting of P (the polynomial block's pivoting):
+//      P_pi = L1 U, pivot nodes pi (m of them), free nodes F (the other
+//      f = n - m); G = L2 L1^-1 (f x m) so that P_F = G P_pi.
+//      Z = [-G^T ; I] (rows pi ; F) spans the null space of P^T.
+//   B  Schur complement S = Z^T A Z = A_FF - G H^T - H G^T (f x f, FP64)
+//      with H = A_Fpi - G A_pipi / 2: one FP64 MFMA product of inner
+//      dimension 2m, [G | H] times [H | G]^T, accumulated onto A_FF.
+//   D  right-hand sides: a particular solution w_p = [P_pi^-T g ; 0] of
+//      the constraints, t = b - A_:pi w_p, y = t_F - G t_pi (f x R, FP64).
+//   C  S rounded to FP32 and factored with partial pivoting (the wave LU of
+//      rbf_operators.h on f rows); v0 = S^-1 y in FP32.
+//   E  FP64 iterative refinement on the scaled reduced system:
+//      r = y - S v (S and y in FP64), v += S32^-1 r, until the estimated
+//      remaining error of the weights is below 1e-9 of their size.
+//   F  w_F = v, w_pi = w_p - G^T v; unscaled and scattered into CSR.
+// A stencil whose refinement does not converge (or whose FP32 factors meet
+// a zero pivot, or whose polynomial block is rank deficient) is solved
+// again from scratch by the plain FP64 wave LU and flagged
+// RBFA_STAT_FALLBACK; its weights are then those of the FP64 form.
+#ifndef RBF_NULLSPACE_H
+#define RBF_NULLSPACE_H
+
+#include "rbf_operators.h"
+
+namespace rbf_operators {
+namespace detail {
+
+typedef double ns_dbl4 __attribute__((ext_vector_type(4)));
+
+template <int DIM, int N, int P, int Q, class Tune = LuTune<0, 2, 0, 0, 1>>
+struct NullSpaceMixed {
+    static constexpr int M = npoly_dim(DIM, P);
+    static constexpr int F = N - M;  // free nodes: order of the reduced system
+    static constexpr int R = DIM + 1;
+    static constexpr int NT = N + M;
+    static constexpr int NC = F + R;  // FP32 row: S plus the right-hand sides
+    static constexpr int nsten = N;
+    static constexpr int T = (F + 15) / 16;  // 16 x 16 tiles per side of S
+    static constexpr int NTILE = T * (T + 1) / 2;  // upper-triangle tiles (S is symmetric)
+    static constexpr int K2 = 2 * M;  // inner dimension of the Schur update
+    static constexpr int KS = (K2 + 3) / 4;  // MFMA k-steps
+    static_assert(N <= 64 && F >= 1, "one node per lane");
+    static_assert(NT <= 64, "the FP64 fallback is the one-wave LU");
+    static_assert(Q >= 3 && Q % 2 == 1, "odd PHS exponent >= 3");
+    using Fb = WaveAssembly<DIM, N, P, Q, R, false, LuTune<4, 2, 2, 1, 1>>;
+    using tune = Tune;
+    // estimated remaining error of the weights, relative to their size, at
+    // which refinement stops (the weights are held to 1e-8 of the FP64
+    // solution; larger systems keep more margin)
+    static constexpr double kTol = NT > 48 ? 2e-10 : 1e-9;
+
+    // LDS (doubles). Region 0 lives through the whole stencil; region 1 holds
+    // the pivot rows of P's LU, A_pipi, t_pi and H until the Schur
+    // product is in registers, then S (FP64, row-major).
+    static constexpr int GS = M + 1;  // row stride of G and H (spreads the banks)
+    static constexpr int SS = F + 1;  // row stride of S
+    static constexpr int o_qf = 0;                   // [DIM][F] free points by rank
+    static constexpr int o_y = o_qf + DIM * F;       // [F][R] reduced right-hand sides
+    static constexpr int o_v = o_y + F * R;          // [F][R] iterate, by unknown
+    static constexpr int o_g = o_v + F * R;          // [F][GS] G by rank
+    static constexpr int o_wp = o_g + F * GS;        // [M][R] particular solution
+    static constexpr int o_int = o_wp + M * R;       // ints: [F] node of each free rank,
+                                                     //       [M] node of each pivot
+    static constexpr int o_pb = (o_int + (F + M + 1) / 2 + 1) / 2 * 2;  // FP32 pivot row
+    static constexpr int o_r1 = o_pb + (NC + 3) / 4 * 2 + 2;
+    static constexpr int o_l = o_r1;                 // [M][M] pivot rows of P's LU
+    static constexpr int o_app = o_l + M * M;        // [M][M] A_pipi
+    static constexpr int o_t = o_app + M * M;        // [M][R] t at the pivot nodes
+    static constexpr int o_h = o_t + M * R;          // [F][GS] H by rank
+    static constexpr int r1_a = o_h + F * GS - o_r1;
+    static constexpr int r1_b = F * SS;
+    static constexpr int o_s = o_r1;
+    static constexpr int ns_doubles = o_r1 + (r1_a > r1_b ? r1_a : r1_b);
+    static constexpr int lds_doubles =
+        (((ns_doubles > Fb::lds_doubles ? ns_doubles : Fb::lds_doubles)) + 1) / 2 * 2;
+
+    // phi(|a - b|) with the squared distance summed as the fill sums it
+    __device__ __forceinline__ static double phi_pair(double ax, double ay, double az, double bx,
+                                                      double by, double bz) {
+#pragma clang fp contract(off)
+        const double dx = ax - bx, dy = ay - by;
+        double r2 = dx * dx + dy * dy;
+        if constexpr (DIM == 3) {
+            const double dz = az - bz;
+            r2 = r2 + dz * dz;
+        }
+        return PHS<Q>{}(r2);
+    }
+
+    template <class Out>
+    __device__ __forceinline__ static void run(const int s_arg, const size_t base_arg,
+                                               const double* __restrict__ x,
+                                               const double* __restrict__ y,
+                                               const double* __restrict__ z,
+                                               const int* __restrict__ ja, Out out, double* lds,
+                                               int* __restrict__ info,
+                                               uint8_t* __restrict__ status, int max_refine) {
+#pragma clang fp contract(off)
+        const int s = __builtin_amdgcn_readfirstlane(s_arg);
+        const size_t base = wave_uniform(base_arg);
+        const int lane = lane_id();
+        const int* ja_row = ja + base;
+        double* qf = lds + o_qf;
+        double* yb = lds + o_y;
+        double* vb = lds + o_v;
+        double* gb = lds + o_g;
+        int* fidx = reinterpret_cast<int*>(lds + o_int);
+        int* pvn = fidx + F;
+        float* pb = reinterpret_cast<float*>(lds + o_pb);
+        double* lb = lds + o_l;
+        double* app = lds + o_app;
+        double* wpb = lds + o_wp;
+        double* tb = lds + o_t;
+        double* hb = lds + o_h;
+        double* sb = lds + o_s;
+
+        // ---- gather, shift, scale (as the FP64 path) ---------------------
+        const int jn = lane < N ? ja_row[lane] : s;
+        const double px = x[jn] - x[s];
+        const double py = y[jn] - y[s];
+        const double pz = DIM == 3 ? z[jn] - z[s] : 0.0;
+        double d2 = px * px + py * py;
+        if constexpr (DIM == 3) d2 = d2 + pz * pz;
+        double rho2 = wave_uniform(wave_max(lane < N ? d2 : 0.0));
+        if (!(rho2 > 0.0)) rho2 = 1.0;
+        const double rho = sqrt(rho2);
+        const double qx = px / rho, qy = py / rho, qz = pz / rho;
+        double mono[M];
+        PolyBasis<P, DIM>{}(qx, qy, qz, mono, 1);
+        // right-hand sides of the kernel rows (the Laplacian and gradient of
+        // phi(|q - q_k|) at the origin)
+        double bk[R];
+        {
+            double r2 = qx * qx + qy * qy;
+            if constexpr (DIM == 3) r2 = r2 + qz * qz;
+            const double tq = PHS<Q - 2>{}(r2);
+            const double gq = static_cast<double>(Q) * tq;
+            bk[0] = static_cast<double>(Q * (Q + DIM - 2)) * tq;
+            bk[1] = -gq * qx;
+            bk[2] = -gq * qy;
+            if constexpr (DIM == 3) bk[3] = -gq * qz;
+        }
+
+        // ---- A: LU with partial pivoting of P, FP64 ------------------------
+        constexpr unsigned long long kRows = N >= 64 ? ~0ull : ((1ull << N) - 1);
+        unsigned long long act = kRows;
+        int pivs[M];
+        int mypt = M;  // pivot step of this lane's node (M: a free node)
+        bool bad = false;
+#pragma unroll
+        for (int t = 0; t < M; ++t) {
+            const double a = mono[t];
+            int p = pivot_lane_mask<2>(a, act);
+            if (p < 0) {
+                bad = true;
+                p = __ffsll(act) - 1;
+            }
+            p = __builtin_amdgcn_readfirstlane(p);
+            const double piv = bcast(a, p);
+            if (piv == 0.0) bad = true;
+            const double rcp = 1.0 / piv;
+            const unsigned long long pm = 1ull << p;
+            const bool elim = lane_in(act & ~pm);
+            const double l = elim ? a * rcp : 0.0;
+#pragma unroll
+            for (int u = t + 1; u < M; ++u) mono[u] = fma(-l, bcast(mono[u], p), mono[u]);
+            mono[t] = elim ? l : a;
+            mypt = lane_in(pm) ? t : mypt;
+            act &= ~pm;
+            pivs[t] = p;
+        }
+        const unsigned long long freem = act;  // free nodes, in lane order
+        const bool isfree = lane_in(freem);
+        const int rank = __builtin_amdgcn_mbcnt_hi(
+            static_cast<unsigned>(freem >> 32),
+            __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(freem), 0u));
+        // pivot rows: L1 below the diagonal, U on and above it
+        if (mypt < M) {
+#pragma unroll
+            for (int u = 0; u < M; ++u) lb[mypt * M + u] = mono[u];
+            pvn[mypt] = lane;
+        }
+        // G = L2 L1^-1 in the free lanes: g L1 = l, from the last column
+#pragma unroll
+        for (int t = M - 2; t >= 0; --t) {
+            double acc = mono[t];
+#pragma unroll
+            for (int u = t + 1; u < M; ++u) acc = fma(-mono[u], bcast(mono[t], pivs[u]), acc);
+            mono[t] = isfree ? acc : mono[t];
+        }
+        // kernel entries against the pivot nodes (the pivot lanes' values
+        // form A_pipi)
+        double akp[M];
+#pragma unroll
+        for (int t = 0; t < M; ++t)
+            akp[t] = phi_pair(qx, qy, qz, bcast(qx, pivs[t]), bcast(qy, pivs[t]),
+                              DIM == 3 ? bcast(qz, pivs[t]) : 0.0);
+        if (mypt < M) {
+#pragma unroll
+            for (int t = 0; t < M; ++t) app[mypt * M + t] = akp[t];
+        }
+        if (isfree) {
+            qf[rank] = qx;
+            qf[F + rank] = qy;
+            if constexpr (DIM == 3) qf[2 * F + rank] = qz;
+            fidx[rank] = lane;
+#pragma unroll
+            for (int t = 0; t < M; ++t) gb[rank * GS + t] = mono[t];
+        }
+        wave_lds_sync();
+
+        // ---- D: particular solution, t, y -----------------------------------
+        // lane c < R solves P_pi^T w = g_c: U^T z = g_c forward, L1^T w = z back
+        if (lane < R) {
+            using Mk = PolyDerivMasks<DIM, P>;
+            const uint64_t gm = lane == 0 ? Mk::lap : (lane == 1 ? Mk::dx : (lane == 2 ? Mk::dy : Mk::dz));
+            const double gv = lane == 0 ? 2.0 : 1.0;
+            double w[M];
+#pragma unroll
+            for (int t = 0; t < M; ++t) {
+                double acc = ((gm >> t) & 1u) ? gv : 0.0;
+#pragma unroll
+                for (int u = 0; u < t; ++u) acc = fma(-lb[u * M + t], w[u], acc);
+                w[t] = acc / lb[t * M + t];
+            }
+#pragma unroll
+            for (int t = M - 2; t >= 0; --t) {
+                double acc = w[t];
+#pragma unroll
+                for (int u = t + 1; u < M; ++u) acc = fma(-lb[u * M + t], w[u], acc);
+                w[t] = acc;
+            }
+#pragma unroll
+            for (int t = 0; t < M; ++t) wpb[t * R + lane] = w[t];
+        }
+        // H = A_Fpi - G A_pipi / 2 (free lanes)
+        {
+            double h[M];
+#pragma unroll
+            for (int t = 0; t < M; ++t) h[t] = 0.0;
+#pragma unroll
+            for (int u = 0; u < M; ++u) {
+#pragma unroll
+                for (int t = 0; t < M; ++t) h[t] = fma(mono[u], app[u * M + t], h[t]);
+            }
+            if (isfree) {
+#pragma unroll
+                for (int t = 0; t < M; ++t) hb[rank * GS + t] = fma(-0.5, h[t], akp[t]);
+            }
+        }
+        wave_lds_sync();
+        // t = b - A_:pi w_p at every node; the pivot nodes' values to LDS
+        double tk[R];
+#pragma unroll
+        for (int c = 0; c < R; ++c) {
+            double acc = bk[c];
+#pragma unroll
+            for (int u = 0; u < M; ++u) acc = fma(-akp[u], wpb[u * R + c], acc);
+            tk[c] = acc;
+        }
+        if (mypt < M) {
+#pragma unroll
+            for (int c = 0; c < R; ++c) tb[mypt * R + c] = tk[c];
+        }
+        wave_lds_sync();
+        if (isfree) {
+#pragma unroll
+            for (int c = 0; c < R; ++c) {
+                double acc = tk[c];
+#pragma unroll
+                for (int u = 0; u < M; ++u) acc = fma(-mono[u], tb[u * R + c], acc);
+                yb[rank * R + c] = acc;
+            }
+        }
+        wave_lds_sync();
+
+        // ---- B: S = A_FF - [G | H] [H | G]^T, FP64 MFMA ---------------------
+        const int li = lane & 15, lg = lane >> 4;
+        ns_dbl4 acc[NTILE];
+        {
+            int ti = 0;
+#pragma unroll
+            for (int I = 0; I < T; ++I) {
+#pragma unroll
+                for (int J = I; J < T; ++J, ++ti) {
+                    const int col = 16 * J + li;
+                    const bool cok = col < F;
+                    const double cx = cok ? qf[col] : 0.0;
+                    const double cy = cok ? qf[F + col] : 0.0;
+                    const double cz = (DIM == 3 && cok) ? qf[2 * F + col] : 0.0;
+                    ns_dbl4 c;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int row = 16 * I + lg + 4 * v;
+                        double e = 0.0;
+                        if (row < F && cok)
+                            e = phi_pair(qf[row], qf[F + row], DIM == 3 ? qf[2 * F + row] : 0.0,
+                                         cx, cy, cz);
+                        c[v] = e;
+                    }
+                    const int ra = 16 * I + li;  // A operand row (free rank)
+#pragma unroll
+                    for (int k = 0; k < KS; ++k) {
+                        const int kk = 4 * k + lg;
+                        double av = 0.0, bv = 0.0;
+                        if (kk < K2) {
+                            // A = -[G | H] rows of tile I; B = [H | G]^T columns of tile J
+                            if (ra < F) av = -(kk < M ? gb[ra * GS + kk] : hb[ra * GS + kk - M]);
+                            if (cok) bv = kk < M ? hb[col * GS + kk] : gb[col * GS + kk - M];
+                        }
+                        c = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, c, 0, 0, 0);
+                    }
+                    acc[ti] = c;
+                }
+            }
+        }
+        wave_lds_sync();  // G/H reads done before S overwrites region 1
+        {
+            int ti = 0;
+#pragma unroll
+            for (int I = 0; I < T; ++I) {
+#pragma unroll
+                for (int J = I; J < T; ++J, ++ti) {
+                    const int col = 16 * J + li;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int row = 16 * I + lg + 4 * v;
+                        if (row < F && col < F) {
+                            sb[row * SS + col] = acc[ti][v];
+                            if (I != J) sb[col * SS + row] = acc[ti][v];
+                        }
+                    }
+                }
+            }
+        }
+        wave_lds_sync();
+
+        // ---- C: FP32 LU of S with the reduced right-hand sides --------------
+        float a32[NC];
+        double yr[R];
+        float rowmax = 0.f;
+#pragma unroll
+        for (int j = 0; j < F; ++j) {
+            a32[j] = lane < F ? static_cast<float>(sb[lane * SS + j]) : 0.f;
+            rowmax = fmaxf(rowmax, fabsf(a32[j]));
+        }
+#pragma unroll
+        for (int c = 0; c < R; ++c) {
+            yr[c] = lane < F ? yb[lane * R + c] : 0.0;
+            a32[F + c] = static_cast<float>(yr[c]);
+        }
+        ElimState<F, float> st;
+        elim_from<0, F, NC, Tune, true>(a32, st, pb, lane);
+        const int mystep = st.mystep;
+        const float myrcp = st.myrcp;
+        // a pivot below 1e-6 of the largest entry of S (duplicated nodes,
+        // nearly singular stencils): the FP32 factors carry no information
+        // and the stencil goes to the FP64 solve
+        const float smax = __uint_as_float(wave_umax(__float_as_uint(rowmax)));
+        const unsigned long long tiny =
+            wave_ballot(lane < F && !(fabsf(myrcp) * (1e-6f * smax) < 1.f));
+        if (tiny) bad = true;
+        // initial solve from the eliminated right-hand sides
+        float r32[R];
+#pragma unroll
+        for (int c = 0; c < R; ++c) r32[c] = a32[F + c];
+#pragma unroll
+        for (int k = F - 1; k >= 0; --k) {
+            const int p = st.perm[k];
+            const float uk = mystep < k ? a32[k] : 0.f;
+#pragma unroll
+            for (int c = 0; c < R; ++c) r32[c] = fmaf(-uk, bcast(r32[c] * myrcp, p), r32[c]);
+        }
+        double vd[R];
+#pragma unroll
+        for (int c = 0; c < R; ++c) vd[c] = static_cast<double>(r32[c] * myrcp);
+
+        // ---- E: FP64 refinement on S v = y ----------------------------------
+        const bool is_row = lane < F;
+        int steps = 0;
+        bool converged = false;
+        float dprev[R];
+#pragma unroll
+        for (int c = 0; c < R; ++c) dprev[c] = 0.f;
+        if (!bad && st.info == 0) {
+            for (int it = 0; it < max_refine; ++it) {
+                if (is_row) {
+#pragma unroll
+                    for (int c = 0; c < R; ++c) vb[mystep * R + c] = vd[c];
+                }
+                wave_lds_sync();
+                double rr[R];
+#pragma unroll
+                for (int c = 0; c < R; ++c) rr[c] = yr[c];
+                if (is_row) {
+#pragma unroll
+                    for (int j = 0; j < F; ++j) {
+                        const double sij = sb[lane * SS + j];
+#pragma unroll
+                        for (int c = 0; c < R; ++c) rr[c] = fma(-sij, vb[j * R + c], rr[c]);
+                    }
+                }
+                wave_lds_sync();
+#pragma unroll
+                for (int c = 0; c < R; ++c) r32[c] = static_cast<float>(rr[c]);
+                // d = U^-1 L^-1 r with the FP32 factors
+#pragma unroll
+                for (int k = 0; k < F; ++k) {
+                    const int p = st.perm[k];
+                    const float lk = mystep > k ? a32[k] : 0.f;
+#pragma unroll
+                    for (int c = 0; c < R; ++c) r32[c] = fmaf(-lk, bcast(r32[c], p), r32[c]);
+                }
+#pragma unroll
+                for (int k = F - 1; k >= 0; --k) {
+                    const int p = st.perm[k];
+                    const float uk = mystep < k ? a32[k] : 0.f;
+#pragma unroll
+                    for (int c = 0; c < R; ++c) r32[c] = fmaf(-uk, bcast(r32[c] * myrcp, p), r32[c]);
+                }
+                ++steps;
+                bool ok = true;
+#pragma unroll
+                for (int c = 0; c < R; ++c) {
+                    const float d = r32[c] * myrcp;
+                    vd[c] = vd[c] + static_cast<double>(d);
+                    const float dm = __uint_as_float(
+                        wave_umax(__float_as_uint(is_row ? fabsf(d) : 0.f)));
+                    const float xm = __uint_as_float(wave_umax(
+                        __float_as_uint(is_row ? static_cast<float>(fabs(vd[c])) : 0.f)));
+                    // remaining error ~ |d| x contraction; the contraction is
+                    // |d| / |v| after the first step (the initial solve's
+                    // relative error), |d_k| / |d_k-1| after later ones
+                    const float rho_c = it == 0 ? (xm > 0.f ? dm / xm : 1.f)
+                                                : (dprev[c] > 0.f ? dm / dprev[c] : 1.f);
+                    if (!(dm * fminf(1.f, rho_c) <= static_cast<float>(kTol) * xm)) ok = false;
+                    dprev[c] = dm;
+                }
+                if (ok) {
+                    converged = true;
+                    break;
+                }
+            }
+        }
+
+        if (!converged) {
+            // the plain FP64 solve of the whole stencil (its own fill, LU,
+            // scatter and info); region 0 and 1 are no longer needed
+            wave_lds_sync();
+            Fb::run(s, base, x, y, z, ja, out, nullptr, nullptr, nullptr, lds, info);
+            if (lane == 0 && status)
+                status[s] = static_cast<uint8_t>((steps & 3) | 4);
+            return;
+        }
+
+        // ---- F: weights --------------------------------------------------------
+        if (is_row) {
+#pragma unroll
+            for (int c = 0; c < R; ++c) vb[mystep * R + c] = vd[c];
+        }
+        wave_lds_sync();
+        const double rho2u = rho2;
+        if (is_row) {
+            const int node = fidx[mystep];
+            const size_t at = base + static_cast<size_t>(node);
+#pragma unroll
+            for (int c = 0; c < R; ++c) out(c)[at] = c == 0 ? vd[c] / rho2u : vd[c] / rho;
+        }
+        if (lane < M) {
+            // w_pi = w_p - G^T v for pivot step t = lane
+            double wsum[R];
+#pragma unroll
+            for (int c = 0; c < R; ++c) wsum[c] = wpb[lane * R + c];
+#pragma unroll
+            for (int j = 0; j < F; ++j) {
+                const double g = gb[j * GS + lane];
+#pragma unroll
+                for (int c = 0; c < R; ++c) wsum[c] = fma(-g, vb[j * R + c], wsum[c]);
+            }
+            const size_t at = base + static_cast<size_t>(pvn[lane]);
+#pragma unroll
+            for (int c = 0; c < R; ++c) out(c)[at] = c == 0 ? wsum[c] / rho2u : wsum[c] / rho;
+        }
+        if (lane == 0) {
+            if (info) info[s] = 0;
+            if (status) status[s] = static_cast<uint8_t>(steps & 3);
+        }
+    }
+};
+
+}  // namespace detail
+}  // namespace rbf_operators
+
+#endif  // RBF_NULLSPACE_H

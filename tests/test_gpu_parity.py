@@ -93,6 +93,54 @@ def test_every_variant_matches_oracle(prec):
                     np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("n,P,Q", [(35, 3, 5), (21, 2, 3)])
+def test_mixed_degenerate_stencils_fall_back(n, P, Q):
+    """Stencils made singular by duplicated nodes: the mixed form must flag
+    them (status FALLBACK) and report the FP64 form's info, which equals the
+    oracle's; every other stencil stays within the mixed bar."""
+    x, y = R.jittered_grid(2, 32, 0.25, seed=9)
+    for k in (100, 400, 777):
+        x[k + 1], y[k + 1] = x[k], y[k]
+    ja = R.knn_stencils(n, x, y)
+    wref, iref = O.assemble(2, n, P, Q, x, y, ja)
+    bad = iref != 0
+    assert bad.any()
+    w64, i64, _ = _run(R.Plan(2, n, P, Q, "diff"), x, y, ja)
+    np.testing.assert_array_equal(i64, iref)
+    plan = R.Plan(2, n, P, Q, "diff", precision="mixed")
+    for v in range(plan.num_variants):
+        plan.variant = v
+        wm, im, st = _run(plan, x, y, ja)
+        miss = np.nonzero(im != iref)[0]
+        e = _relerr(wm, wref, n)[~bad]
+        print(f"variant {v} ({plan.variant_name()}): {bad.sum()} singular stencils, "
+              f"fallback {((st & 4) != 0).sum()}, info mismatches {miss.tolist()[:8]} "
+              f"(status {st[miss].tolist()[:8]}, oracle info {iref[miss].tolist()[:8]}), "
+              f"max rel err elsewhere {e.max():.3e}")
+        np.testing.assert_array_equal(im, iref)
+        assert ((st[bad] & 4) != 0).all(), plan.variant_name()
+        assert e.max() <= MIXED_TOL
+
+
+def test_mixed_null_space_converges_in_one_step():
+    """The headline shape (n = 35, r^5 + P3) on the null-space variant: no
+    stencil falls back and almost every stencil stops after one FP64
+    refinement step."""
+    n, P, Q = 35, 3, 5
+    plan = R.Plan(2, n, P, Q, "diff", precision="mixed")
+    assert plan.variant_name(0).startswith("null space")
+    x, y = R.jittered_grid(2, 96, 0.25, seed=15)
+    ja = R.knn_stencils(n, x, y)
+    w, info, st = _run(plan, x, y, ja)
+    wref, _ = O.assemble(2, n, P, Q, x, y, ja)
+    assert (info == 0).all()
+    hist = np.bincount(st & 3, minlength=4)
+    print("refinement steps histogram", hist.tolist(), "fallback", int(((st & 4) != 0).sum()))
+    assert ((st & 4) == 0).all()
+    assert hist[1] >= 0.9 * st.size
+    assert _relerr(w, wref, n).max() <= MIXED_TOL
+
+
 def test_autotune_picks_a_variant():
     import torch
     n, P, Q = 21, 2, 3
