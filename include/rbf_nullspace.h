@@ -10,29 +10,32 @@
 // 2e7 at n = 56, r^7 + P4), but almost all of that comes from the
 // polynomial block. Restricted to the null space of P^T the kernel block is
 // well conditioned (3e3 and 4e4 for the same stencils), so FP32 factors of
-// the restriction converge under FP64 refinement in one or two steps where
-// FP32 factors of the whole system take three, or never converge.
+// the restriction converge under FP64 refinement in one step where FP32
+// factors of the whole system take two or three, or never converge.
 //
 // Per stencil:
 //   A  FP64 LU with partial pivoting of P (the polynomial block's pivoting):
 //      P_pi = L1 U, pivot nodes pi (m of them), free nodes F (the other
-//      f = n - m); G = L2 L1^-1 (f x m) so that P_F = G P_pi.
+//      f = n - m, in lane order); G = L2 L1^-1 (f x m) so that P_F = G P_pi.
 //      Z = [-G^T ; I] (rows pi ; F) spans the null space of P^T.
 //   B  Schur complement S = Z^T A Z = A_FF - G H^T - H G^T (f x f, FP64)
 //      with H = A_Fpi - G A_pipi / 2: one FP64 MFMA product of inner
 //      dimension 2m, [G | H] times [H | G]^T, accumulated onto A_FF.
 //   D  right-hand sides: a particular solution w_p = [P_pi^-T g ; 0] of
 //      the constraints, t = b - A_:pi w_p, y = t_F - G t_pi (f x R, FP64).
-//   C  S rounded to FP32 and factored with partial pivoting (the wave LU of
-//      rbf_operators.h on f rows); v0 = S^-1 y in FP32.
+//   C  S rounded to FP32 and factored without pivoting (S is definite for
+//      the shipped kernel / degree pairs: r^Q with polynomial degree at least
+//      (Q - 1) / 2 makes Z^T A Z (-1)^((Q+1)/2)-definite), row k on lane k;
+//      v0 = S^-1 y in FP32.
 //   E  FP64 iterative refinement on the scaled reduced system:
 //      r = y - S v (S and y in FP64), v += S32^-1 r, until the estimated
-//      remaining error of the weights is below 1e-9 of their size.
+//      remaining error of the weights is below tol of their size.
 //   F  w_F = v, w_pi = w_p - G^T v; unscaled and scattered into CSR.
-// A stencil whose refinement does not converge (or whose FP32 factors meet
-// a zero pivot, or whose polynomial block is rank deficient) is solved
-// again from scratch by the plain FP64 wave LU and flagged
-// RBFA_STAT_FALLBACK; its weights are then those of the FP64 form.
+// A stencil whose refinement does not converge, whose FP32 factors meet a
+// pivot below 1e-6 of the largest entry of S (duplicated nodes), or whose
+// polynomial block is rank deficient is solved again from scratch by the
+// plain FP64 wave LU and flagged RBFA_STAT_FALLBACK; its weights and info
+// are then those of the FP64 form.
 #ifndef RBF_NULLSPACE_H
 #define RBF_NULLSPACE_H
 
@@ -43,7 +46,7 @@ namespace detail {
 
 typedef double ns_dbl4 __attribute__((ext_vector_type(4)));
 
-template <int DIM, int N, int P, int Q, class Tune = LuTune<0, 2, 0, 0, 1>>
+template <int DIM, int N, int P, int Q, class Tune = LuTune<4, 2>>
 struct NullSpaceMixed {
     static constexpr int M = npoly_dim(DIM, P);
     static constexpr int F = N - M;  // free nodes: order of the reduced system
@@ -52,6 +55,7 @@ struct NullSpaceMixed {
     static constexpr int NC = F + R;  // FP32 row: S plus the right-hand sides
     static constexpr int nsten = N;
     static constexpr int T = (F + 15) / 16;  // 16 x 16 tiles per side of S
+    static constexpr int TP = 16 * T;        // padded order of S
     static constexpr int NTILE = T * (T + 1) / 2;  // upper-triangle tiles (S is symmetric)
     static constexpr int K2 = 2 * M;  // inner dimension of the Schur update
     static constexpr int KS = (K2 + 3) / 4;  // MFMA k-steps
@@ -66,24 +70,26 @@ struct NullSpaceMixed {
     static constexpr double kTol = NT > 48 ? 2e-10 : 1e-9;
 
     // LDS (doubles). Region 0 lives through the whole stencil; region 1 holds
-    // the pivot rows of P's LU, A_pipi, t_pi and H until the Schur
-    // product is in registers, then S (FP64, row-major).
+    // the pivot rows of P's LU, A_pipi, t_pi and H until the Schur product is
+    // in registers, then S (FP64, row-major).
     static constexpr int GS = M + 1;  // row stride of G and H (spreads the banks)
     static constexpr int SS = F + 1;  // row stride of S
-    static constexpr int o_qf = 0;                   // [DIM][F] free points by rank
-    static constexpr int o_y = o_qf + DIM * F;       // [F][R] reduced right-hand sides
+    static constexpr int o_qf = 0;                   // [DIM][TP] free points by rank (0-padded)
+    static constexpr int o_y = o_qf + DIM * TP;      // [F][R] reduced right-hand sides
     static constexpr int o_v = o_y + F * R;          // [F][R] iterate, by unknown
-    static constexpr int o_g = o_v + F * R;          // [F][GS] G by rank
-    static constexpr int o_wp = o_g + F * GS;        // [M][R] particular solution
-    static constexpr int o_int = o_wp + M * R;       // ints: [F] node of each free rank,
+    static constexpr int o_d = o_v + F * R;          // [F][R] FP32 solution of a solve
+    static constexpr int o_g = o_d + (F * R + 1) / 2;  // [TP][GS] G by rank (0-padded)
+    static constexpr int o_wp = o_g + TP * GS;       // [M][R] particular solution
+    static constexpr int o_ru = o_wp + M * R;        // [M] 1 / U_tt of P's LU
+    static constexpr int o_int = o_ru + M;           // ints: [F] node of each free rank,
                                                      //       [M] node of each pivot
     static constexpr int o_pb = (o_int + (F + M + 1) / 2 + 1) / 2 * 2;  // FP32 pivot row
     static constexpr int o_r1 = o_pb + (NC + 3) / 4 * 2 + 2;
     static constexpr int o_l = o_r1;                 // [M][M] pivot rows of P's LU
     static constexpr int o_app = o_l + M * M;        // [M][M] A_pipi
     static constexpr int o_t = o_app + M * M;        // [M][R] t at the pivot nodes
-    static constexpr int o_h = o_t + M * R;          // [F][GS] H by rank
-    static constexpr int r1_a = o_h + F * GS - o_r1;
+    static constexpr int o_h = o_t + M * R;          // [TP][GS] H by rank (0-padded)
+    static constexpr int r1_a = o_h + TP * GS - o_r1;
     static constexpr int r1_b = F * SS;
     static constexpr int o_s = o_r1;
     static constexpr int ns_doubles = o_r1 + (r1_a > r1_b ? r1_a : r1_b);
@@ -103,6 +109,44 @@ struct NullSpaceMixed {
         return PHS<Q>{}(r2);
     }
 
+    // Elimination step K of the reduced system without pivoting: row K is
+    // lane K. Rows below K store their multiplier in a[K]; rows K and above
+    // keep their U entries.
+    template <int K>
+    __device__ __forceinline__ static void elim_np(float (&a)[NC], float& myrcp, float* pb) {
+        if constexpr (K < F) {
+            constexpr unsigned long long rows = F >= 64 ? ~0ull : ((1ull << F) - 1);
+            constexpr unsigned long long below = rows & ~((2ull << K) - 1);
+            const float piv = bcast(a[K], K);
+            const float rcp = __builtin_amdgcn_rcpf(piv);
+            const bool elim = lane_in(below);
+            const bool me = lane_in(1ull << K);
+            const float l = elim ? a[K] * rcp : 0.f;
+            a[K] = elim ? l : a[K];
+            myrcp = me ? rcp : myrcp;
+            asm volatile("" : "+v"(myrcp), "+v"(a[K]));
+            constexpr int BCK = K < 5 ? 0 : Tune::BC;
+            update_cols<K + 1, BCK>(a, l, K, pb, me);
+            elim_np<K + 1>(a, myrcp, pb);
+        }
+    }
+
+    // Back substitution of U x = r (the FP32 factors, r one value per row
+    // lane): x_k is read from lane k and stored to d (all lanes store the
+    // same value); rows at and below k then take garbage, already consumed.
+    __device__ __forceinline__ static void back_np(const float (&a)[NC], float myrcp,
+                                                   float (&r)[R], float* d) {
+#pragma unroll
+        for (int k = F - 1; k >= 0; --k) {
+#pragma unroll
+            for (int c = 0; c < R; ++c) {
+                const float xk = bcast(r[c] * myrcp, k);
+                d[k * R + c] = xk;
+                r[c] = fmaf(-a[k], xk, r[c]);
+            }
+        }
+    }
+
     template <class Out>
     __device__ __forceinline__ static void run(const int s_arg, const size_t base_arg,
                                                const double* __restrict__ x,
@@ -119,13 +163,15 @@ struct NullSpaceMixed {
         double* qf = lds + o_qf;
         double* yb = lds + o_y;
         double* vb = lds + o_v;
+        float* db = reinterpret_cast<float*>(lds + o_d);
         double* gb = lds + o_g;
+        double* wpb = lds + o_wp;
+        double* rub = lds + o_ru;
         int* fidx = reinterpret_cast<int*>(lds + o_int);
         int* pvn = fidx + F;
         float* pb = reinterpret_cast<float*>(lds + o_pb);
         double* lb = lds + o_l;
         double* app = lds + o_app;
-        double* wpb = lds + o_wp;
         double* tb = lds + o_t;
         double* hb = lds + o_h;
         double* sb = lds + o_s;
@@ -174,7 +220,7 @@ struct NullSpaceMixed {
             p = __builtin_amdgcn_readfirstlane(p);
             const double piv = bcast(a, p);
             if (piv == 0.0) bad = true;
-            const double rcp = 1.0 / piv;
+            const double rcp = recip_uniform(piv);
             const unsigned long long pm = 1ull << p;
             const bool elim = lane_in(act & ~pm);
             const double l = elim ? a * rcp : 0.0;
@@ -182,6 +228,7 @@ struct NullSpaceMixed {
             for (int u = t + 1; u < M; ++u) mono[u] = fma(-l, bcast(mono[u], p), mono[u]);
             mono[t] = elim ? l : a;
             mypt = lane_in(pm) ? t : mypt;
+            if (lane == 0) rub[t] = rcp;
             act &= ~pm;
             pivs[t] = p;
         }
@@ -196,14 +243,6 @@ struct NullSpaceMixed {
             for (int u = 0; u < M; ++u) lb[mypt * M + u] = mono[u];
             pvn[mypt] = lane;
         }
-        // G = L2 L1^-1 in the free lanes: g L1 = l, from the last column
-#pragma unroll
-        for (int t = M - 2; t >= 0; --t) {
-            double acc = mono[t];
-#pragma unroll
-            for (int u = t + 1; u < M; ++u) acc = fma(-mono[u], bcast(mono[t], pivs[u]), acc);
-            mono[t] = isfree ? acc : mono[t];
-        }
         // kernel entries against the pivot nodes (the pivot lanes' values
         // form A_pipi)
         double akp[M];
@@ -215,15 +254,34 @@ struct NullSpaceMixed {
 #pragma unroll
             for (int t = 0; t < M; ++t) app[mypt * M + t] = akp[t];
         }
+        // padding ranks F .. TP-1: zero points, zero G and H rows
+        if (lane < TP - F) {
+#pragma unroll
+            for (int a = 0; a < DIM; ++a) qf[a * TP + F + lane] = 0.0;
+#pragma unroll
+            for (int t = 0; t < M; ++t) {
+                gb[(F + lane) * GS + t] = 0.0;
+                hb[(F + lane) * GS + t] = 0.0;
+            }
+        }
+        wave_lds_sync();
+        // G = L2 L1^-1: g L1 = l, from the last column (L1 read from LDS; the
+        // pivot lanes compute values nobody reads)
+#pragma unroll
+        for (int t = M - 2; t >= 0; --t) {
+            double acc = mono[t];
+#pragma unroll
+            for (int u = t + 1; u < M; ++u) acc = fma(-mono[u], lb[u * M + t], acc);
+            mono[t] = acc;
+        }
         if (isfree) {
             qf[rank] = qx;
-            qf[F + rank] = qy;
-            if constexpr (DIM == 3) qf[2 * F + rank] = qz;
+            qf[TP + rank] = qy;
+            if constexpr (DIM == 3) qf[2 * TP + rank] = qz;
             fidx[rank] = lane;
 #pragma unroll
             for (int t = 0; t < M; ++t) gb[rank * GS + t] = mono[t];
         }
-        wave_lds_sync();
 
         // ---- D: particular solution, t, y -----------------------------------
         // lane c < R solves P_pi^T w = g_c: U^T z = g_c forward, L1^T w = z back
@@ -237,7 +295,7 @@ struct NullSpaceMixed {
                 double acc = ((gm >> t) & 1u) ? gv : 0.0;
 #pragma unroll
                 for (int u = 0; u < t; ++u) acc = fma(-lb[u * M + t], w[u], acc);
-                w[t] = acc / lb[t * M + t];
+                w[t] = acc * rub[t];
             }
 #pragma unroll
             for (int t = M - 2; t >= 0; --t) {
@@ -288,9 +346,9 @@ struct NullSpaceMixed {
                 yb[rank * R + c] = acc;
             }
         }
-        wave_lds_sync();
 
         // ---- B: S = A_FF - [G | H] [H | G]^T, FP64 MFMA ---------------------
+        // (padding ranks have zero G and H rows; their entries are ignored)
         const int li = lane & 15, lg = lane >> 4;
         ns_dbl4 acc[NTILE];
         {
@@ -300,29 +358,34 @@ struct NullSpaceMixed {
 #pragma unroll
                 for (int J = I; J < T; ++J, ++ti) {
                     const int col = 16 * J + li;
-                    const bool cok = col < F;
-                    const double cx = cok ? qf[col] : 0.0;
-                    const double cy = cok ? qf[F + col] : 0.0;
-                    const double cz = (DIM == 3 && cok) ? qf[2 * F + col] : 0.0;
+                    const double cx = qf[col], cy = qf[TP + col];
+                    const double cz = DIM == 3 ? qf[2 * TP + col] : 0.0;
                     ns_dbl4 c;
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const int row = 16 * I + lg + 4 * v;
-                        double e = 0.0;
-                        if (row < F && cok)
-                            e = phi_pair(qf[row], qf[F + row], DIM == 3 ? qf[2 * F + row] : 0.0,
-                                         cx, cy, cz);
-                        c[v] = e;
+                        c[v] = phi_pair(qf[row], qf[TP + row], DIM == 3 ? qf[2 * TP + row] : 0.0,
+                                        cx, cy, cz);
                     }
                     const int ra = 16 * I + li;  // A operand row (free rank)
 #pragma unroll
                     for (int k = 0; k < KS; ++k) {
                         const int kk = 4 * k + lg;
-                        double av = 0.0, bv = 0.0;
-                        if (kk < K2) {
-                            // A = -[G | H] rows of tile I; B = [H | G]^T columns of tile J
-                            if (ra < F) av = -(kk < M ? gb[ra * GS + kk] : hb[ra * GS + kk - M]);
-                            if (cok) bv = kk < M ? hb[col * GS + kk] : gb[col * GS + kk - M];
+                        // A = -[G | H] rows of tile I; B = [H | G]^T columns of tile J
+                        double av, bv;
+                        if (4 * k + 3 < M) {
+                            av = -gb[ra * GS + kk];
+                            bv = hb[col * GS + kk];
+                        } else if (4 * k >= M && 4 * k + 3 < K2) {
+                            av = -hb[ra * GS + kk - M];
+                            bv = gb[col * GS + kk - M];
+                        } else {
+                            const bool lo = kk < M;
+                            const int kq = lo ? kk : kk - M;
+                            const double* ga = lo ? gb : hb;
+                            const double* gbb = lo ? hb : gb;
+                            av = kk < K2 ? -ga[ra * GS + kq] : 0.0;
+                            bv = kk < K2 ? gbb[col * GS + kq] : 0.0;
                         }
                         c = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, c, 0, 0, 0);
                     }
@@ -351,58 +414,49 @@ struct NullSpaceMixed {
         }
         wave_lds_sync();
 
-        // ---- C: FP32 LU of S with the reduced right-hand sides --------------
+        // ---- C: FP32 LU of S (no pivoting) with the reduced right-hand sides
+        const bool is_row = lane < F;
         float a32[NC];
         double yr[R];
         float rowmax = 0.f;
 #pragma unroll
         for (int j = 0; j < F; ++j) {
-            a32[j] = lane < F ? static_cast<float>(sb[lane * SS + j]) : 0.f;
+            a32[j] = is_row ? static_cast<float>(sb[lane * SS + j]) : 0.f;
             rowmax = fmaxf(rowmax, fabsf(a32[j]));
         }
 #pragma unroll
         for (int c = 0; c < R; ++c) {
-            yr[c] = lane < F ? yb[lane * R + c] : 0.0;
+            yr[c] = is_row ? yb[lane * R + c] : 0.0;
             a32[F + c] = static_cast<float>(yr[c]);
         }
-        ElimState<F, float> st;
-        elim_from<0, F, NC, Tune, true>(a32, st, pb, lane);
-        const int mystep = st.mystep;
-        const float myrcp = st.myrcp;
+        float myrcp = 0.f;
+        elim_np<0>(a32, myrcp, pb);
         // a pivot below 1e-6 of the largest entry of S (duplicated nodes,
         // nearly singular stencils): the FP32 factors carry no information
         // and the stencil goes to the FP64 solve
         const float smax = __uint_as_float(wave_umax(__float_as_uint(rowmax)));
-        const unsigned long long tiny =
-            wave_ballot(lane < F && !(fabsf(myrcp) * (1e-6f * smax) < 1.f));
-        if (tiny) bad = true;
+        if (wave_ballot(is_row && !(fabsf(myrcp) * (1e-6f * smax) < 1.f))) bad = true;
         // initial solve from the eliminated right-hand sides
         float r32[R];
 #pragma unroll
         for (int c = 0; c < R; ++c) r32[c] = a32[F + c];
-#pragma unroll
-        for (int k = F - 1; k >= 0; --k) {
-            const int p = st.perm[k];
-            const float uk = mystep < k ? a32[k] : 0.f;
-#pragma unroll
-            for (int c = 0; c < R; ++c) r32[c] = fmaf(-uk, bcast(r32[c] * myrcp, p), r32[c]);
-        }
+        back_np(a32, myrcp, r32, db);
+        wave_lds_sync();
         double vd[R];
 #pragma unroll
-        for (int c = 0; c < R; ++c) vd[c] = static_cast<double>(r32[c] * myrcp);
+        for (int c = 0; c < R; ++c) vd[c] = is_row ? static_cast<double>(db[lane * R + c]) : 0.0;
 
         // ---- E: FP64 refinement on S v = y ----------------------------------
-        const bool is_row = lane < F;
         int steps = 0;
         bool converged = false;
         float dprev[R];
 #pragma unroll
         for (int c = 0; c < R; ++c) dprev[c] = 0.f;
-        if (!bad && st.info == 0) {
+        if (!bad) {
             for (int it = 0; it < max_refine; ++it) {
                 if (is_row) {
 #pragma unroll
-                    for (int c = 0; c < R; ++c) vb[mystep * R + c] = vd[c];
+                    for (int c = 0; c < R; ++c) vb[lane * R + c] = vd[c];
                 }
                 wave_lds_sync();
                 double rr[R];
@@ -416,32 +470,26 @@ struct NullSpaceMixed {
                         for (int c = 0; c < R; ++c) rr[c] = fma(-sij, vb[j * R + c], rr[c]);
                     }
                 }
-                wave_lds_sync();
 #pragma unroll
                 for (int c = 0; c < R; ++c) r32[c] = static_cast<float>(rr[c]);
-                // d = U^-1 L^-1 r with the FP32 factors
+                // L y = r: rows below k take l_ik y_k
 #pragma unroll
-                for (int k = 0; k < F; ++k) {
-                    const int p = st.perm[k];
-                    const float lk = mystep > k ? a32[k] : 0.f;
+                for (int k = 0; k < F - 1; ++k) {
+                    constexpr unsigned long long rows = F >= 64 ? ~0ull : ((1ull << F) - 1);
+                    const unsigned long long below = rows & ~((2ull << k) - 1);
+                    const float lk = lane_in(below) ? a32[k] : 0.f;
 #pragma unroll
-                    for (int c = 0; c < R; ++c) r32[c] = fmaf(-lk, bcast(r32[c], p), r32[c]);
+                    for (int c = 0; c < R; ++c) r32[c] = fmaf(-lk, bcast(r32[c], k), r32[c]);
                 }
-#pragma unroll
-                for (int k = F - 1; k >= 0; --k) {
-                    const int p = st.perm[k];
-                    const float uk = mystep < k ? a32[k] : 0.f;
-#pragma unroll
-                    for (int c = 0; c < R; ++c) r32[c] = fmaf(-uk, bcast(r32[c] * myrcp, p), r32[c]);
-                }
+                back_np(a32, myrcp, r32, db);
+                wave_lds_sync();
                 ++steps;
                 bool ok = true;
 #pragma unroll
                 for (int c = 0; c < R; ++c) {
-                    const float d = r32[c] * myrcp;
+                    const float d = is_row ? db[lane * R + c] : 0.f;
                     vd[c] = vd[c] + static_cast<double>(d);
-                    const float dm = __uint_as_float(
-                        wave_umax(__float_as_uint(is_row ? fabsf(d) : 0.f)));
+                    const float dm = __uint_as_float(wave_umax(__float_as_uint(fabsf(d))));
                     const float xm = __uint_as_float(wave_umax(
                         __float_as_uint(is_row ? static_cast<float>(fabs(vd[c])) : 0.f)));
                     // remaining error ~ |d| x contraction; the contraction is
@@ -464,24 +512,19 @@ struct NullSpaceMixed {
             // scatter and info); region 0 and 1 are no longer needed
             wave_lds_sync();
             Fb::run(s, base, x, y, z, ja, out, nullptr, nullptr, nullptr, lds, info);
-            if (lane == 0 && status)
-                status[s] = static_cast<uint8_t>((steps & 3) | 4);
+            if (lane == 0 && status) status[s] = static_cast<uint8_t>((steps & 3) | 4);
             return;
         }
 
         // ---- F: weights --------------------------------------------------------
         if (is_row) {
 #pragma unroll
-            for (int c = 0; c < R; ++c) vb[mystep * R + c] = vd[c];
+            for (int c = 0; c < R; ++c) vb[lane * R + c] = vd[c];
+            const size_t at = base + static_cast<size_t>(fidx[lane]);
+#pragma unroll
+            for (int c = 0; c < R; ++c) out(c)[at] = c == 0 ? vd[c] / rho2 : vd[c] / rho;
         }
         wave_lds_sync();
-        const double rho2u = rho2;
-        if (is_row) {
-            const int node = fidx[mystep];
-            const size_t at = base + static_cast<size_t>(node);
-#pragma unroll
-            for (int c = 0; c < R; ++c) out(c)[at] = c == 0 ? vd[c] / rho2u : vd[c] / rho;
-        }
         if (lane < M) {
             // w_pi = w_p - G^T v for pivot step t = lane
             double wsum[R];
@@ -495,7 +538,7 @@ struct NullSpaceMixed {
             }
             const size_t at = base + static_cast<size_t>(pvn[lane]);
 #pragma unroll
-            for (int c = 0; c < R; ++c) out(c)[at] = c == 0 ? wsum[c] / rho2u : wsum[c] / rho;
+            for (int c = 0; c < R; ++c) out(c)[at] = c == 0 ? wsum[c] / rho2 : wsum[c] / rho;
         }
         if (lane == 0) {
             if (info) info[s] = 0;
