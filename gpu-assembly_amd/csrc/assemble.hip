@@ -199,12 +199,12 @@ struct NsMixVariants {
     template <class T>
     using WM = detail::WaveMixed<DIM, N, P, Q, T>;
     static constexpr Variant list[4] = {
+        {"null space: fp64 poly lu + mfma schur, fp32 symmetric lu occ=4",
+         &MixL<NS<detail::LuTune<1, 2>>, 1, 4>::run},
         {"null space: fp64 poly lu + mfma schur, fp32 lu bc=readlane occ=4",
-         &MixL<NS<detail::LuTune<0, 2, 0, 0, 1>>, 1, 4>::run},
-        {"null space: fp64 poly lu + mfma schur, fp32 lu bc=third occ=4",
-         &MixL<NS<detail::LuTune<4, 2, 0, 0, 1>>, 1, 4>::run},
-        {"null space: fp64 poly lu + mfma schur, fp32 lu bc=readlane occ=3",
-         &MixL<NS<detail::LuTune<0, 2, 0, 0, 1>>, 1, 3>::run},
+         &MixL<NS<detail::LuTune<0, 2>>, 1, 4>::run},
+        {"null space: fp64 poly lu + mfma schur, fp32 symmetric lu occ=3",
+         &MixL<NS<detail::LuTune<1, 2>>, 1, 3>::run},
         {"whole system: fp32 lu bc=third pv=dpp-bcast pin fill=pairs",
          &MixL<WM<detail::LuTune<4, 2, 1, 0, 1>>, 1, 1>::run},
     };

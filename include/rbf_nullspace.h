@@ -25,7 +25,8 @@
 //      the constraints, t = b - A_:pi w_p, y = t_F - G t_pi (f x R, FP64).
 //   C  S rounded to FP32 and factored without pivoting (S is definite for
 //      the shipped kernel / degree pairs: r^Q with polynomial degree at least
-//      (Q - 1) / 2 makes Z^T A Z (-1)^((Q+1)/2)-definite), row k on lane k;
+//      (Q - 1) / 2 makes Z^T A Z (-1)^((Q+1)/2)-definite), row k on lane k,
+//      the pivot row taken from the published column (S stays symmetric);
 //      v0 = S^-1 y in FP32.
 //   E  FP64 iterative refinement on the scaled reduced system:
 //      r = y - S v (S and y in FP64), v += S32^-1 r, until the estimated
@@ -67,26 +68,32 @@ struct NullSpaceMixed {
     // estimated remaining error of the weights, relative to their size, at
     // which refinement stops (the weights are held to 1e-8 of the FP64
     // solution; larger systems keep more margin)
-    static constexpr double kTol = NT > 48 ? 2e-10 : 1e-9;
+    static constexpr double kTol = NT > 48 ? 1e-10 : 3e-10;
 
-    // LDS (doubles). Region 0 lives through the whole stencil; region 1 holds
-    // the pivot rows of P's LU, A_pipi, t_pi and H until the Schur product is
-    // in registers, then S (FP64, row-major).
+    // LDS (doubles). Region 0 lives through the whole stencil and ends with
+    // the f rows of G; region 1 follows it. In phases A-D region 1 holds the
+    // pivot rows of P's LU, A_pipi, t_pi and H; the padding rows of G
+    // (ranks f .. TP-1, zero, read only by the MFMA products) take the
+    // place of P's pivot rows once the particular solution has used them;
+    // after the Schur product region 1 holds S (FP64, row-major).
     static constexpr int GS = M + 1;  // row stride of G and H (spreads the banks)
-    static constexpr int SS = F + 1;  // row stride of S
+    static constexpr int SS = F;      // row stride of S
+    static constexpr int CB = 64;  // floats per published column (every lane stores)
     static constexpr int o_qf = 0;                   // [DIM][TP] free points by rank (0-padded)
     static constexpr int o_y = o_qf + DIM * TP;      // [F][R] reduced right-hand sides
     static constexpr int o_v = o_y + F * R;          // [F][R] iterate, by unknown
-    static constexpr int o_d = o_v + F * R;          // [F][R] FP32 solution of a solve
-    static constexpr int o_g = o_d + (F * R + 1) / 2;  // [TP][GS] G by rank (0-padded)
-    static constexpr int o_wp = o_g + TP * GS;       // [M][R] particular solution
+    static constexpr int o_wp = o_v + F * R;         // [M][R] particular solution
     static constexpr int o_ru = o_wp + M * R;        // [M] 1 / U_tt of P's LU
     static constexpr int o_int = o_ru + M;           // ints: [F] node of each free rank,
                                                      //       [M] node of each pivot
-    static constexpr int o_pb = (o_int + (F + M + 1) / 2 + 1) / 2 * 2;  // FP32 pivot row
-    static constexpr int o_r1 = o_pb + (NC + 3) / 4 * 2 + 2;
+    // FP32 pivot row (elim_np) or two published columns (elim_sym)
+    static constexpr int o_pb = (o_int + (F + M + 1) / 2 + 1) / 2 * 2;
+    static constexpr int pb_doubles = (NC + 3) / 4 * 2 > CB ? (NC + 3) / 4 * 2 : CB;
+    static constexpr int o_g = o_pb + pb_doubles;    // [TP][GS] G by rank (rows >= F in region 1)
+    static constexpr int o_r1 = o_g + F * GS;
     static constexpr int o_l = o_r1;                 // [M][M] pivot rows of P's LU
-    static constexpr int o_app = o_l + M * M;        // [M][M] A_pipi
+    static constexpr int l_span = M * M > (TP - F) * GS ? M * M : (TP - F) * GS;
+    static constexpr int o_app = o_l + l_span;       // [M][M] A_pipi
     static constexpr int o_t = o_app + M * M;        // [M][R] t at the pivot nodes
     static constexpr int o_h = o_t + M * R;          // [TP][GS] H by rank (0-padded)
     static constexpr int r1_a = o_h + TP * GS - o_r1;
@@ -131,18 +138,78 @@ struct NullSpaceMixed {
         }
     }
 
-    // Back substitution of U x = r (the FP32 factors, r one value per row
-    // lane): x_k is read from lane k and stored to d (all lanes store the
-    // same value); rows at and below k then take garbage, already consumed.
-    __device__ __forceinline__ static void back_np(const float (&a)[NC], float myrcp,
-                                                   float (&r)[R], float* d) {
+    // The same step using the symmetry of S: the trailing matrix of a
+    // symmetric matrix stays symmetric under elimination without pivoting,
+    // so row K of it equals column K, which every lane publishes at once
+    // (one LDS store per lane) and reads back at uniform addresses, instead
+    // of a v_readlane per pivot-row entry. The right-hand sides still come
+    // from lane K. Rows K and above keep their own (row) values as U.
+    template <int K>
+    __device__ __forceinline__ static void elim_sym(float (&a)[NC], float& myrcp, float* cb) {
+        if constexpr (K < F) {
+            constexpr unsigned long long rows = F >= 64 ? ~0ull : ((1ull << F) - 1);
+            constexpr unsigned long long below = rows & ~((2ull << K) - 1);
+            float* col = cb + (K & 1) * CB;  // two buffers taken in turn
+            col[lane_id()] = a[K];
+            const float piv = bcast(a[K], K);
+            const float rcp = __builtin_amdgcn_rcpf(piv);
+            const bool elim = lane_in(below);
+            const bool me = lane_in(1ull << K);
+            const float l = elim ? a[K] * rcp : 0.f;
+            a[K] = elim ? l : a[K];
+            myrcp = me ? rcp : myrcp;
+            asm volatile("" : "+v"(myrcp), "+v"(a[K]));
+            float u[R];
 #pragma unroll
-        for (int k = F - 1; k >= 0; --k) {
+            for (int c = 0; c < R; ++c) u[c] = bcast(a[F + c], K);
+            wave_lds_sync();
 #pragma unroll
-            for (int c = 0; c < R; ++c) {
-                const float xk = bcast(r[c] * myrcp, k);
-                d[k * R + c] = xk;
-                r[c] = fmaf(-a[k], xk, r[c]);
+            for (int j = K + 1; j < F; ++j) a[j] = fmaf(-l, col[j], a[j]);
+#pragma unroll
+            for (int c = 0; c < R; ++c) a[F + c] = fmaf(-l, u[c], a[F + c]);
+            elim_sym<K + 1>(a, myrcp, cb);
+        }
+    }
+
+    // Rows i < K of the factors divided by their pivot at column K, so U
+    // has a unit diagonal (its reciprocal pivots are applied to the
+    // right-hand sides instead): row i's U entries sit at columns j > i.
+    template <int K>
+    __device__ __forceinline__ static void unit_u(float (&a)[NC], float myrcp) {
+        if constexpr (K < F) {
+            constexpr unsigned long long lt = (1ull << K) - 1;  // lanes (rows) < K
+            if (lane_in(lt)) a[K] = a[K] * myrcp;
+            unit_u<K + 1>(a, myrcp);
+        }
+    }
+
+    // U x = r with the unit-diagonal factors, r already scaled by the row's
+    // reciprocal pivot: x_k is row k's value once the rows below have been
+    // folded in; rows above take -u_ik x_k. Each row ends holding its x.
+    __device__ __forceinline__ static void back_unit(const float (&a)[NC], float (&r)[R]) {
+#pragma unroll
+        for (int k = F - 1; k > 0; --k) {
+            float xk[R];
+#pragma unroll
+            for (int c = 0; c < R; ++c) xk[c] = bcast(r[c], k);
+            if (lane_in((1ull << k) - 1)) {
+#pragma unroll
+                for (int c = 0; c < R; ++c) r[c] = fmaf(-a[k], xk[c], r[c]);
+            }
+        }
+    }
+
+    // L y = r with the stored multipliers: rows below k take -l_ik y_k.
+    __device__ __forceinline__ static void fwd_unit(const float (&a)[NC], float (&r)[R]) {
+        constexpr unsigned long long rows = F >= 64 ? ~0ull : ((1ull << F) - 1);
+#pragma unroll
+        for (int k = 0; k < F - 1; ++k) {
+            float yk[R];
+#pragma unroll
+            for (int c = 0; c < R; ++c) yk[c] = bcast(r[c], k);
+            if (lane_in(rows & ~((2ull << k) - 1))) {
+#pragma unroll
+                for (int c = 0; c < R; ++c) r[c] = fmaf(-a[k], yk[c], r[c]);
             }
         }
     }
@@ -163,7 +230,6 @@ struct NullSpaceMixed {
         double* qf = lds + o_qf;
         double* yb = lds + o_y;
         double* vb = lds + o_v;
-        float* db = reinterpret_cast<float*>(lds + o_d);
         double* gb = lds + o_g;
         double* wpb = lds + o_wp;
         double* rub = lds + o_ru;
@@ -186,7 +252,9 @@ struct NullSpaceMixed {
         double rho2 = wave_uniform(wave_max(lane < N ? d2 : 0.0));
         if (!(rho2 > 0.0)) rho2 = 1.0;
         const double rho = sqrt(rho2);
-        const double qx = px / rho, qy = py / rho, qz = pz / rho;
+        const double irho = wave_uniform(1.0 / rho);
+        const double irho2 = irho * irho;
+        const double qx = px * irho, qy = py * irho, qz = pz * irho;
         double mono[M];
         PolyBasis<P, DIM>{}(qx, qy, qz, mono, 1);
         // right-hand sides of the kernel rows (the Laplacian and gradient of
@@ -212,21 +280,24 @@ struct NullSpaceMixed {
 #pragma unroll
         for (int t = 0; t < M; ++t) {
             const double a = mono[t];
-            int p = pivot_lane_mask<2>(a, act);
-            if (p < 0) {
-                bad = true;
-                p = __ffsll(act) - 1;
-            }
-            p = __builtin_amdgcn_readfirstlane(p);
+            // pivot: the largest |a| among the rows still active, compared by
+            // its high word (lowest lane among equal high words: any of them
+            // serves the conditioning); NaN wins and sends the stencil to the
+            // FP64 solve through the checks below
+            const unsigned key =
+                lane_in(act) ? (static_cast<unsigned>(__double2hiint(a)) & 0x7fffffffu) : 0u;
+            const unsigned mk = wave_umax_bcast(key);
+            const int p = __ffsll(act & wave_ballot(key == mk)) - 1;
             const double piv = bcast(a, p);
-            if (piv == 0.0) bad = true;
+            if (!(piv != 0.0)) bad = true;
             const double rcp = recip_uniform(piv);
             const unsigned long long pm = 1ull << p;
-            const bool elim = lane_in(act & ~pm);
-            const double l = elim ? a * rcp : 0.0;
+            if (lane_in(act & ~pm)) {
+                const double l = a * rcp;
 #pragma unroll
-            for (int u = t + 1; u < M; ++u) mono[u] = fma(-l, bcast(mono[u], p), mono[u]);
-            mono[t] = elim ? l : a;
+                for (int u = t + 1; u < M; ++u) mono[u] = fma(-l, bcast(mono[u], p), mono[u]);
+                mono[t] = l;
+            }
             mypt = lane_in(pm) ? t : mypt;
             if (lane == 0) rub[t] = rcp;
             act &= ~pm;
@@ -254,15 +325,12 @@ struct NullSpaceMixed {
 #pragma unroll
             for (int t = 0; t < M; ++t) app[mypt * M + t] = akp[t];
         }
-        // padding ranks F .. TP-1: zero points, zero G and H rows
+        // padding ranks F .. TP-1: zero points and H rows (G's below)
         if (lane < TP - F) {
 #pragma unroll
             for (int a = 0; a < DIM; ++a) qf[a * TP + F + lane] = 0.0;
 #pragma unroll
-            for (int t = 0; t < M; ++t) {
-                gb[(F + lane) * GS + t] = 0.0;
-                hb[(F + lane) * GS + t] = 0.0;
-            }
+            for (int t = 0; t < M; ++t) hb[(F + lane) * GS + t] = 0.0;
         }
         wave_lds_sync();
         // G = L2 L1^-1: g L1 = l, from the last column (L1 read from LDS; the
@@ -280,7 +348,10 @@ struct NullSpaceMixed {
             if constexpr (DIM == 3) qf[2 * TP + rank] = qz;
             fidx[rank] = lane;
 #pragma unroll
-            for (int t = 0; t < M; ++t) gb[rank * GS + t] = mono[t];
+            for (int t = 0; t < M; ++t) {
+                gb[rank * GS + t] = mono[t];
+                hb[rank * GS + t] = akp[t];
+            }
         }
 
         // ---- D: particular solution, t, y -----------------------------------
@@ -307,19 +378,34 @@ struct NullSpaceMixed {
 #pragma unroll
             for (int t = 0; t < M; ++t) wpb[t * R + lane] = w[t];
         }
-        // H = A_Fpi - G A_pipi / 2 (free lanes)
+        wave_lds_sync();  // P's pivot rows read: G's padding rows take their place
+        if (lane < TP - F) {
+#pragma unroll
+            for (int t = 0; t < M; ++t) gb[(F + lane) * GS + t] = 0.0;
+        }
+        wave_lds_sync();
+        // H = A_Fpi - G A_pipi / 2 on FP64 MFMA: tile I of free ranks, the
+        // m pivot columns (C = A_Fpi from hb, A = -G/2, B = A_pipi)
         {
-            double h[M];
+            const int li = lane & 15, lg = lane >> 4;
+            constexpr int KH = (M + 3) / 4;
 #pragma unroll
-            for (int t = 0; t < M; ++t) h[t] = 0.0;
+            for (int I = 0; I < T; ++I) {
+                ns_dbl4 c;
 #pragma unroll
-            for (int u = 0; u < M; ++u) {
+                for (int v = 0; v < 4; ++v)
+                    c[v] = li < M ? hb[(16 * I + lg + 4 * v) * GS + li] : 0.0;
 #pragma unroll
-                for (int t = 0; t < M; ++t) h[t] = fma(mono[u], app[u * M + t], h[t]);
-            }
-            if (isfree) {
+                for (int k = 0; k < KH; ++k) {
+                    const int kk = 4 * k + lg;
+                    const double av = kk < M ? -0.5 * gb[(16 * I + li) * GS + kk] : 0.0;
+                    const double bv = (kk < M && li < M) ? app[kk * M + li] : 0.0;
+                    c = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, c, 0, 0, 0);
+                }
+                if (li < M) {
 #pragma unroll
-                for (int t = 0; t < M; ++t) hb[rank * GS + t] = fma(-0.5, h[t], akp[t]);
+                    for (int v = 0; v < 4; ++v) hb[(16 * I + lg + 4 * v) * GS + li] = c[v];
+                }
             }
         }
         wave_lds_sync();
@@ -430,21 +516,26 @@ struct NullSpaceMixed {
             a32[F + c] = static_cast<float>(yr[c]);
         }
         float myrcp = 0.f;
-        elim_np<0>(a32, myrcp, pb);
+        if constexpr (Tune::BC == 1) {
+            elim_sym<0>(a32, myrcp, pb);
+            wave_lds_sync();  // column reads done before the buffer is reused
+        } else {
+            elim_np<0>(a32, myrcp, pb);
+        }
         // a pivot below 1e-6 of the largest entry of S (duplicated nodes,
         // nearly singular stencils): the FP32 factors carry no information
         // and the stencil goes to the FP64 solve
         const float smax = __uint_as_float(wave_umax(__float_as_uint(rowmax)));
         if (wave_ballot(is_row && !(fabsf(myrcp) * (1e-6f * smax) < 1.f))) bad = true;
+        unit_u<1>(a32, myrcp);
         // initial solve from the eliminated right-hand sides
         float r32[R];
 #pragma unroll
-        for (int c = 0; c < R; ++c) r32[c] = a32[F + c];
-        back_np(a32, myrcp, r32, db);
-        wave_lds_sync();
+        for (int c = 0; c < R; ++c) r32[c] = a32[F + c] * myrcp;
+        back_unit(a32, r32);
         double vd[R];
 #pragma unroll
-        for (int c = 0; c < R; ++c) vd[c] = is_row ? static_cast<double>(db[lane * R + c]) : 0.0;
+        for (int c = 0; c < R; ++c) vd[c] = is_row ? static_cast<double>(r32[c]) : 0.0;
 
         // ---- E: FP64 refinement on S v = y ----------------------------------
         int steps = 0;
@@ -472,22 +563,15 @@ struct NullSpaceMixed {
                 }
 #pragma unroll
                 for (int c = 0; c < R; ++c) r32[c] = static_cast<float>(rr[c]);
-                // L y = r: rows below k take l_ik y_k
+                fwd_unit(a32, r32);
 #pragma unroll
-                for (int k = 0; k < F - 1; ++k) {
-                    constexpr unsigned long long rows = F >= 64 ? ~0ull : ((1ull << F) - 1);
-                    const unsigned long long below = rows & ~((2ull << k) - 1);
-                    const float lk = lane_in(below) ? a32[k] : 0.f;
-#pragma unroll
-                    for (int c = 0; c < R; ++c) r32[c] = fmaf(-lk, bcast(r32[c], k), r32[c]);
-                }
-                back_np(a32, myrcp, r32, db);
-                wave_lds_sync();
+                for (int c = 0; c < R; ++c) r32[c] = r32[c] * myrcp;
+                back_unit(a32, r32);
                 ++steps;
                 bool ok = true;
 #pragma unroll
                 for (int c = 0; c < R; ++c) {
-                    const float d = is_row ? db[lane * R + c] : 0.f;
+                    const float d = is_row ? r32[c] : 0.f;
                     vd[c] = vd[c] + static_cast<double>(d);
                     const float dm = __uint_as_float(wave_umax(__float_as_uint(fabsf(d))));
                     const float xm = __uint_as_float(wave_umax(
@@ -522,7 +606,7 @@ struct NullSpaceMixed {
             for (int c = 0; c < R; ++c) vb[lane * R + c] = vd[c];
             const size_t at = base + static_cast<size_t>(fidx[lane]);
 #pragma unroll
-            for (int c = 0; c < R; ++c) out(c)[at] = c == 0 ? vd[c] / rho2 : vd[c] / rho;
+            for (int c = 0; c < R; ++c) out(c)[at] = vd[c] * (c == 0 ? irho2 : irho);
         }
         wave_lds_sync();
         if (lane < M) {
@@ -538,7 +622,7 @@ struct NullSpaceMixed {
             }
             const size_t at = base + static_cast<size_t>(pvn[lane]);
 #pragma unroll
-            for (int c = 0; c < R; ++c) out(c)[at] = c == 0 ? wsum[c] / rho2 : wsum[c] / rho;
+            for (int c = 0; c < R; ++c) out(c)[at] = wsum[c] * (c == 0 ? irho2 : irho);
         }
         if (lane == 0) {
             if (info) info[s] = 0;
