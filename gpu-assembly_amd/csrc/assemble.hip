@@ -198,13 +198,15 @@ struct NsMixVariants {
     using NS = detail::NullSpaceMixed<DIM, N, P, Q, T>;
     template <class T>
     using WM = detail::WaveMixed<DIM, N, P, Q, T>;
-    static constexpr Variant list[4] = {
-        {"null space: fp64 poly lu + mfma schur, fp32 symmetric lu occ=4",
-         &MixL<NS<detail::LuTune<1, 2>>, 1, 4>::run},
+    static constexpr Variant list[5] = {
         {"null space: fp64 poly lu + mfma schur, fp32 lu bc=readlane occ=4",
-         &MixL<NS<detail::LuTune<0, 2>>, 1, 4>::run},
-        {"null space: fp64 poly lu + mfma schur, fp32 symmetric lu occ=3",
-         &MixL<NS<detail::LuTune<1, 2>>, 1, 3>::run},
+         &MixL<NS<detail::NsTune<0>>, 1, 4>::run},
+        {"null space: fp64 poly lu + mfma schur, fp32 lu 8 symmetric steps occ=4",
+         &MixL<NS<detail::NsTune<8>>, 1, 4>::run},
+        {"null space: fp64 poly lu + mfma schur, fp32 lu 16 symmetric steps occ=4",
+         &MixL<NS<detail::NsTune<16>>, 1, 4>::run},
+        {"null space: fp64 poly lu + mfma schur, fp32 symmetric lu occ=4",
+         &MixL<NS<detail::NsTune<64>>, 1, 4>::run},
         {"whole system: fp32 lu bc=third pv=dpp-bcast pin fill=pairs",
          &MixL<WM<detail::LuTune<4, 2, 1, 0, 1>>, 1, 1>::run},
     };
@@ -213,7 +215,7 @@ template <int DIM, int N, int P, int Q>
 constexpr Shape diffmix() {
     using E = detail::WaveMixed<DIM, N, P, Q>;
     return {DIM, N, P, Q, RBFA_OPS_DIFF, 0, RBFA_PREC_MIXED, E::NT, DIM + 1,
-            NsMixVariants<DIM, N, P, Q>::list, 4};
+            NsMixVariants<DIM, N, P, Q>::list, 5};
 }
 
 // The shapes this library ships: the stencil families of the workloads it

@@ -47,7 +47,15 @@ namespace detail {
 
 typedef double ns_dbl4 __attribute__((ext_vector_type(4)));
 
-template <int DIM, int N, int P, int Q, class Tune = LuTune<4, 2>>
+// Knob of the reduced FP32 LU: its first SYM elimination steps take the
+// pivot row from the published column (symmetry of S, LDS), the rest by
+// v_readlane from the pivot lane.
+template <int SYM_>
+struct NsTune {
+    static constexpr int SYM = SYM_;
+};
+
+template <int DIM, int N, int P, int Q, class Tune = NsTune<0>>
 struct NullSpaceMixed {
     static constexpr int M = npoly_dim(DIM, P);
     static constexpr int F = N - M;  // free nodes: order of the reduced system
@@ -103,7 +111,29 @@ struct NullSpaceMixed {
     static constexpr int lds_doubles =
         (((ns_doubles > Fb::lds_doubles ? ns_doubles : Fb::lds_doubles)) + 1) / 2 * 2;
 
-    // phi(|a - b|) with the squared distance summed as the fill sums it
+    // r^E from r^2 for odd E: the v_rsq_f64 seed refined to the square
+    // root as the compiler's own expansion does, without its rescaling of
+    // denormal inputs and its special cases (stencil distances are normal
+    // or exactly zero; zero gives zero)
+    template <int E>
+    __device__ __forceinline__ static double rpow(double r2) {
+#pragma clang fp contract(off)
+        const double y = __builtin_amdgcn_rsq(r2);
+        double g = r2 * y, h = 0.5 * y;
+        const double e = fma(-h, g, 0.5);
+        g = fma(g, e, g);
+        h = fma(h, e, h);
+        double d = fma(-g, g, r2);
+        g = fma(d, h, g);
+        d = fma(-g, g, r2);
+        g = fma(d, h, g);
+        double p = g;
+#pragma unroll
+        for (int i = 0; i < (E - 1) / 2; ++i) p = p * r2;
+        return r2 == 0.0 ? 0.0 : p;
+    }
+
+    // phi(|a - b|) = |a - b|^Q
     __device__ __forceinline__ static double phi_pair(double ax, double ay, double az, double bx,
                                                       double by, double bz) {
 #pragma clang fp contract(off)
@@ -113,7 +143,7 @@ struct NullSpaceMixed {
             const double dz = az - bz;
             r2 = r2 + dz * dz;
         }
-        return PHS<Q>{}(r2);
+        return rpow<Q>(r2);
     }
 
     // Elimination step K of the reduced system without pivoting: row K is
@@ -132,8 +162,7 @@ struct NullSpaceMixed {
             a[K] = elim ? l : a[K];
             myrcp = me ? rcp : myrcp;
             asm volatile("" : "+v"(myrcp), "+v"(a[K]));
-            constexpr int BCK = K < 5 ? 0 : Tune::BC;
-            update_cols<K + 1, BCK>(a, l, K, pb, me);
+            update_cols<K + 1, 0>(a, l, K, pb, me);
             elim_np<K + 1>(a, myrcp, pb);
         }
     }
@@ -146,7 +175,10 @@ struct NullSpaceMixed {
     // from lane K. Rows K and above keep their own (row) values as U.
     template <int K>
     __device__ __forceinline__ static void elim_sym(float (&a)[NC], float& myrcp, float* cb) {
-        if constexpr (K < F) {
+        if constexpr (K >= Tune::SYM) {
+            wave_lds_sync();  // column reads done before the buffer is reused
+            elim_np<K>(a, myrcp, cb);
+        } else if constexpr (K < F) {
             constexpr unsigned long long rows = F >= 64 ? ~0ull : ((1ull << F) - 1);
             constexpr unsigned long long below = rows & ~((2ull << K) - 1);
             float* col = cb + (K & 1) * CB;  // two buffers taken in turn
@@ -263,7 +295,7 @@ struct NullSpaceMixed {
         {
             double r2 = qx * qx + qy * qy;
             if constexpr (DIM == 3) r2 = r2 + qz * qz;
-            const double tq = PHS<Q - 2>{}(r2);
+            const double tq = rpow<Q - 2>(r2);
             const double gq = static_cast<double>(Q) * tq;
             bk[0] = static_cast<double>(Q * (Q + DIM - 2)) * tq;
             bk[1] = -gq * qx;
@@ -516,12 +548,7 @@ struct NullSpaceMixed {
             a32[F + c] = static_cast<float>(yr[c]);
         }
         float myrcp = 0.f;
-        if constexpr (Tune::BC == 1) {
-            elim_sym<0>(a32, myrcp, pb);
-            wave_lds_sync();  // column reads done before the buffer is reused
-        } else {
-            elim_np<0>(a32, myrcp, pb);
-        }
+        elim_sym<0>(a32, myrcp, pb);
         // a pivot below 1e-6 of the largest entry of S (duplicated nodes,
         // nearly singular stencils): the FP32 factors carry no information
         // and the stencil goes to the FP64 solve
@@ -534,8 +561,12 @@ struct NullSpaceMixed {
         for (int c = 0; c < R; ++c) r32[c] = a32[F + c] * myrcp;
         back_unit(a32, r32);
         double vd[R];
+        float xm[R];  // size of each operator's weights, for the stopping test
 #pragma unroll
-        for (int c = 0; c < R; ++c) vd[c] = is_row ? static_cast<double>(r32[c]) : 0.0;
+        for (int c = 0; c < R; ++c) {
+            vd[c] = is_row ? static_cast<double>(r32[c]) : 0.0;
+            xm[c] = __uint_as_float(wave_umax(__float_as_uint(is_row ? fabsf(r32[c]) : 0.f)));
+        }
 
         // ---- E: FP64 refinement on S v = y ----------------------------------
         int steps = 0;
@@ -574,14 +605,12 @@ struct NullSpaceMixed {
                     const float d = is_row ? r32[c] : 0.f;
                     vd[c] = vd[c] + static_cast<double>(d);
                     const float dm = __uint_as_float(wave_umax(__float_as_uint(fabsf(d))));
-                    const float xm = __uint_as_float(wave_umax(
-                        __float_as_uint(is_row ? static_cast<float>(fabs(vd[c])) : 0.f)));
                     // remaining error ~ |d| x contraction; the contraction is
                     // |d| / |v| after the first step (the initial solve's
                     // relative error), |d_k| / |d_k-1| after later ones
-                    const float rho_c = it == 0 ? (xm > 0.f ? dm / xm : 1.f)
+                    const float rho_c = it == 0 ? (xm[c] > 0.f ? dm / xm[c] : 1.f)
                                                 : (dprev[c] > 0.f ? dm / dprev[c] : 1.f);
-                    if (!(dm * fminf(1.f, rho_c) <= static_cast<float>(kTol) * xm)) ok = false;
+                    if (!(dm * fminf(1.f, rho_c) <= static_cast<float>(kTol) * xm[c])) ok = false;
                     dprev[c] = dm;
                 }
                 if (ok) {
