@@ -281,7 +281,11 @@ struct NullSpaceMixed {
         const double pz = DIM == 3 ? z[jn] - z[s] : 0.0;
         double d2 = px * px + py * py;
         if constexpr (DIM == 3) d2 = d2 + pz * pz;
-        double rho2 = wave_uniform(wave_max(lane < N ? d2 : 0.0));
+        // the stencil radius: the node with the largest squared distance by
+        // its high word (the scaling only needs a radius of the right size)
+        const unsigned dkey = lane < N ? static_cast<unsigned>(__double2hiint(d2)) : 0u;
+        const unsigned dmax = wave_umax_bcast(dkey);
+        double rho2 = bcast(d2, __ffsll(wave_ballot(dkey == dmax)) - 1);
         if (!(rho2 > 0.0)) rho2 = 1.0;
         const double rho = sqrt(rho2);
         const double irho = wave_uniform(1.0 / rho);
@@ -306,7 +310,6 @@ struct NullSpaceMixed {
         // ---- A: LU with partial pivoting of P, FP64 ------------------------
         constexpr unsigned long long kRows = N >= 64 ? ~0ull : ((1ull << N) - 1);
         unsigned long long act = kRows;
-        int pivs[M];
         int mypt = M;  // pivot step of this lane's node (M: a free node)
         bool bad = false;
 #pragma unroll
@@ -322,7 +325,9 @@ struct NullSpaceMixed {
             const int p = __ffsll(act & wave_ballot(key == mk)) - 1;
             const double piv = bcast(a, p);
             if (!(piv != 0.0)) bad = true;
-            const double rcp = recip_uniform(piv);
+            double rcp = __builtin_amdgcn_rcp(piv);  // two Newton steps from the seed
+            rcp = fma(rcp, fma(-piv, rcp, 1.0), rcp);
+            rcp = fma(rcp, fma(-piv, rcp, 1.0), rcp);
             const unsigned long long pm = 1ull << p;
             if (lane_in(act & ~pm)) {
                 const double l = a * rcp;
@@ -333,26 +338,29 @@ struct NullSpaceMixed {
             mypt = lane_in(pm) ? t : mypt;
             if (lane == 0) rub[t] = rcp;
             act &= ~pm;
-            pivs[t] = p;
         }
         const unsigned long long freem = act;  // free nodes, in lane order
         const bool isfree = lane_in(freem);
         const int rank = __builtin_amdgcn_mbcnt_hi(
             static_cast<unsigned>(freem >> 32),
             __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(freem), 0u));
-        // pivot rows: L1 below the diagonal, U on and above it
+        // pivot rows: L1 below the diagonal, U on and above it; the pivot
+        // nodes' points (in t_pi's place, which is written later)
         if (mypt < M) {
 #pragma unroll
             for (int u = 0; u < M; ++u) lb[mypt * M + u] = mono[u];
             pvn[mypt] = lane;
+            tb[mypt * R] = qx;
+            tb[mypt * R + 1] = qy;
+            if constexpr (DIM == 3) tb[mypt * R + 2] = qz;
         }
+        wave_lds_sync();
         // kernel entries against the pivot nodes (the pivot lanes' values
         // form A_pipi)
         double akp[M];
 #pragma unroll
         for (int t = 0; t < M; ++t)
-            akp[t] = phi_pair(qx, qy, qz, bcast(qx, pivs[t]), bcast(qy, pivs[t]),
-                              DIM == 3 ? bcast(qz, pivs[t]) : 0.0);
+            akp[t] = phi_pair(qx, qy, qz, tb[t * R], tb[t * R + 1], DIM == 3 ? tb[t * R + 2] : 0.0);
         if (mypt < M) {
 #pragma unroll
             for (int t = 0; t < M; ++t) app[mypt * M + t] = akp[t];
