@@ -41,6 +41,11 @@ template <int DIM, int N, int P, int Q>
 hipError_t launch_group_diff(const LaunchArgs& a);
 template <int DIM, int N, int P, int Q>
 hipError_t launch_group_mixed(const LaunchArgs& a);
+// The null-space mixed form for systems larger than 64 unknowns whose
+// stencils still fit a wave (assemble_group_mixed.hip): SYM symmetric
+// steps of the reduced LU, OCC waves per SIMD.
+template <int DIM, int N, int P, int Q, int SYM, int OCC>
+hipError_t launch_ns_list(const LaunchArgs& a);
 // Systems of 64 unknowns and more, blocked LU with FP64 MFMA trailing
 // updates, one workgroup per stencil, one wave per 16-column tile
 // (assemble_tile.hip).

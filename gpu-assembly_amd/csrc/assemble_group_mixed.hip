@@ -6,6 +6,7 @@
 
 #include "assemble_common.h"
 #include "rbf_group.h"
+#include "rbf_nullspace.h"
 #include "rbf_tile.h"
 
 namespace rbf_amd_internal {
@@ -66,6 +67,43 @@ hipError_t launch_group_mixed(const LaunchArgs& a) {
     return hipGetLastError();
 }
 
+// Null-space mixed form of systems larger than a wavefront: one wave per
+// stencil (n <= 64 nodes, n - m <= 64 free nodes), stencils that do not
+// converge listed for the blocked-LU FP64 pass.
+template <class E, int OCC>
+__global__ void __launch_bounds__(64, OCC)
+k_assemble_ns_list(int nstencils, const double* __restrict__ x, const double* __restrict__ y,
+                   const double* __restrict__ z, const int* __restrict__ ja,
+                   detail::ValueOut<double, E::R> out, int* __restrict__ info,
+                   uint8_t* __restrict__ status, int max_refine, int* __restrict__ ws) {
+    __shared__ __align__(16) double lds[E::lds_doubles];
+    const int s = xcd_block(nstencils);
+    if (s >= nstencils) return;
+    E::run(s, static_cast<size_t>(s) * E::nsten, x, y, z, ja, out, lds, info, status, max_refine,
+           ws, ws + 1);
+}
+
+template <int DIM, int N, int P, int Q, int SYM, int OCC>
+hipError_t launch_ns_list(const LaunchArgs& a) {
+    using E = detail::NullSpaceMixed<DIM, N, P, Q, detail::NsTune<SYM>>;
+    using G = detail::TileAssembly<DIM, N, P, Q>;
+    detail::ValueOut<double, E::R> out;
+    for (int c = 0; c < E::R; ++c) out.p[c] = a.w[c];
+    hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(64), 0, a.stream, a.ws);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_assemble_ns_list<E, OCC>), dim3(xcd_grid(a.nstencils)), dim3(64), 0,
+                       a.stream, a.nstencils, a.x, a.y, a.z, a.ja, out, a.info, a.status,
+                       a.max_refine, a.ws);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_group_fallback<G>), dim3(1024), dim3(G::threads), 0, a.stream, a.x, a.y,
+                       a.z, a.ja, out, a.info, a.ws);
+    return hipGetLastError();
+}
+
+template hipError_t launch_ns_list<2, 56, 4, 7, 16, 1>(const LaunchArgs&);
+template hipError_t launch_ns_list<2, 56, 4, 7, 64, 1>(const LaunchArgs&);
 template hipError_t launch_group_mixed<2, 56, 4, 7>(const LaunchArgs&);
 template hipError_t launch_group_mixed<3, 80, 3, 3>(const LaunchArgs&);
 

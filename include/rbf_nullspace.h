@@ -34,9 +34,10 @@
 //   F  w_F = v, w_pi = w_p - G^T v; unscaled and scattered into CSR.
 // A stencil whose refinement does not converge, whose FP32 factors meet a
 // pivot below 1e-6 of the largest entry of S (duplicated nodes), or whose
-// polynomial block is rank deficient is solved again from scratch by the
-// plain FP64 wave LU and flagged RBFA_STAT_FALLBACK; its weights and info
-// are then those of the FP64 form.
+// polynomial block is rank deficient is solved again from scratch in FP64
+// and flagged RBFA_STAT_FALLBACK; its weights and info are then those of
+// the FP64 form. Systems of up to 64 unknowns take the wave LU in the same
+// wave; larger ones are listed (fb_count, fb_list) for the blocked LU.
 #ifndef RBF_NULLSPACE_H
 #define RBF_NULLSPACE_H
 
@@ -69,9 +70,16 @@ struct NullSpaceMixed {
     static constexpr int K2 = 2 * M;  // inner dimension of the Schur update
     static constexpr int KS = (K2 + 3) / 4;  // MFMA k-steps
     static_assert(N <= 64 && F >= 1, "one node per lane");
-    static_assert(NT <= 64, "the FP64 fallback is the one-wave LU");
     static_assert(Q >= 3 && Q % 2 == 1, "odd PHS exponent >= 3");
+    // Fallback: systems of up to 64 unknowns are solved again in the same
+    // wave by the FP64 wave LU; larger ones are appended to a list that a
+    // second launch solves with the FP64 blocked LU.
+    static constexpr bool kInWaveFallback = NT <= 64;
     using Fb = WaveAssembly<DIM, N, P, Q, R, false, LuTune<4, 2, 2, 1, 1>>;
+    static constexpr int fb_lds = [] {
+        if constexpr (kInWaveFallback) return Fb::lds_doubles;
+        else return 0;
+    }();
     using tune = Tune;
     // estimated remaining error of the weights, relative to their size, at
     // which refinement stops (the weights are held to 1e-8 of the FP64
@@ -108,8 +116,7 @@ struct NullSpaceMixed {
     static constexpr int r1_b = F * SS;
     static constexpr int o_s = o_r1;
     static constexpr int ns_doubles = o_r1 + (r1_a > r1_b ? r1_a : r1_b);
-    static constexpr int lds_doubles =
-        (((ns_doubles > Fb::lds_doubles ? ns_doubles : Fb::lds_doubles)) + 1) / 2 * 2;
+    static constexpr int lds_doubles = ((ns_doubles > fb_lds ? ns_doubles : fb_lds) + 1) / 2 * 2;
 
     // r^E from r^2 for odd E: the v_rsq_f64 seed refined to the square
     // root as the compiler's own expansion does, without its rescaling of
@@ -253,7 +260,9 @@ struct NullSpaceMixed {
                                                const double* __restrict__ z,
                                                const int* __restrict__ ja, Out out, double* lds,
                                                int* __restrict__ info,
-                                               uint8_t* __restrict__ status, int max_refine) {
+                                               uint8_t* __restrict__ status, int max_refine,
+                                               int* __restrict__ fb_count = nullptr,
+                                               int* __restrict__ fb_list = nullptr) {
 #pragma clang fp contract(off)
         const int s = __builtin_amdgcn_readfirstlane(s_arg);
         const size_t base = wave_uniform(base_arg);
@@ -629,10 +638,14 @@ struct NullSpaceMixed {
         }
 
         if (!converged) {
-            // the plain FP64 solve of the whole stencil (its own fill, LU,
-            // scatter and info); region 0 and 1 are no longer needed
-            wave_lds_sync();
-            Fb::run(s, base, x, y, z, ja, out, nullptr, nullptr, nullptr, lds, info);
+            if constexpr (kInWaveFallback) {
+                // the plain FP64 solve of the whole stencil (its own fill,
+                // LU, scatter and info); regions 0 and 1 are no longer needed
+                wave_lds_sync();
+                Fb::run(s, base, x, y, z, ja, out, nullptr, nullptr, nullptr, lds, info);
+            } else if (lane == 0) {
+                fb_list[atomicAdd(fb_count, 1)] = s;  // solved by the FP64 pass
+            }
             if (lane == 0 && status) status[s] = static_cast<uint8_t>((steps & 3) | 4);
             return;
         }

@@ -307,3 +307,5 @@ def test_large_systems_mixed(dim, m, n, P, Q):
     assert e.max() <= MIXED_TOL
     if fb.any():
         assert e[fb].max() <= FP64_TOL
+    # the default variants converge: FP64 fallback stays the exception
+    assert fb.mean() <= 0.05
