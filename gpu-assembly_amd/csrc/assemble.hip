@@ -214,12 +214,12 @@ struct NsMixVariants {
     template <class T>
     using WM = detail::WaveMixed<DIM, N, P, Q, T>;
     static constexpr Variant list[5] = {
+        {"null space: fp64 poly lu + mfma schur, fp32 lu 16 symmetric steps occ=4",
+         &MixL<NS<detail::NsTune<16>>, 1, 4>::run},
         {"null space: fp64 poly lu + mfma schur, fp32 lu bc=readlane occ=4",
          &MixL<NS<detail::NsTune<0>>, 1, 4>::run},
         {"null space: fp64 poly lu + mfma schur, fp32 lu 8 symmetric steps occ=4",
          &MixL<NS<detail::NsTune<8>>, 1, 4>::run},
-        {"null space: fp64 poly lu + mfma schur, fp32 lu 16 symmetric steps occ=4",
-         &MixL<NS<detail::NsTune<16>>, 1, 4>::run},
         {"null space: fp64 poly lu + mfma schur, fp32 symmetric lu occ=4",
          &MixL<NS<detail::NsTune<64>>, 1, 4>::run},
         {"whole system: fp32 lu bc=third pv=dpp-bcast pin fill=pairs",
@@ -373,6 +373,10 @@ RBFA_API int32_t rbfa_plan_autotune(rbfa_plan* plan, int32_t nstencils, const do
     int best = plan->variant;
     float best_ms = 0.f;
     int32_t rc = RBFA_OK;
+    // untimed launches first, so the first variant timed does not carry the
+    // device's ramp from idle
+    for (int r = 0; r < reps + 1 && rc == RBFA_OK; ++r)
+        rc = rbfa_assemble(plan, nstencils, x, y, z, ja, w, xc, yc, zc, info, status, stream);
     for (int v = 0; v < plan->shape.nvariants && rc == RBFA_OK; ++v) {
         plan->variant = v;
         rc = rbfa_assemble(plan, nstencils, x, y, z, ja, w, xc, yc, zc, info, status, stream);
@@ -389,9 +393,9 @@ RBFA_API int32_t rbfa_plan_autotune(rbfa_plan* plan, int32_t nstencils, const do
         }
         ms /= reps;
         if (ms_out) ms_out[v] = ms;
-        // a later variant must win by more than 1%: differences inside the
+        // a later variant must win by more than 2%: differences inside the
         // run-to-run spread keep the earlier (default) variant
-        if (v == 0 || ms < 0.99f * best_ms) {
+        if (v == 0 || ms < 0.98f * best_ms) {
             best = v;
             best_ms = ms;
         }

@@ -240,8 +240,9 @@ RBFA_API int32_t rbfa_plan_get_variant(const rbfa_plan* plan);
 RBFA_API int32_t rbfa_plan_set_variant(rbfa_plan* plan, int32_t variant);
 /* Time every variant on the given inputs (reps launches each, HIP events on
  * `stream`, synchronising it), keep the fastest in the plan (a later
- * variant must beat the best earlier one by more than 1%, so run-to-run
- * noise keeps the lower-numbered variant), and write the
+ * variant must beat the best earlier one by more than 2%, so run-to-run
+ * noise keeps the lower-numbered variant; the current variant runs reps+1
+ * times untimed first), and write the
  * per-variant milliseconds to ms_out[num_variants] if non-NULL. The
  * outputs w/info/status are overwritten. */
 RBFA_API int32_t rbfa_plan_autotune(rbfa_plan* plan, int32_t nstencils, const double* x,

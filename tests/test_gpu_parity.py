@@ -150,8 +150,8 @@ def test_autotune_picks_a_variant():
     w = [torch.empty(ja.size, dtype=torch.float64, device="cuda:0") for _ in range(3)]
     ms = plan.autotune(x.size, _dev(x), _dev(y), _dev(ja), w, reps=2)
     assert len(ms) == plan.num_variants and all(t > 0 for t in ms)
-    # the kept variant is within the autotuner's 1% noise margin of the fastest
-    assert ms[plan.variant] <= min(ms) / 0.99 * (1 + 1e-6)
+    # the kept variant is within the autotuner's 2% noise margin of the fastest
+    assert ms[plan.variant] <= min(ms) / 0.98 * (1 + 1e-6)
 
 
 def test_interp_d2q9_matches_oracle():
