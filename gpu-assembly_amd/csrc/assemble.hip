@@ -13,7 +13,7 @@
 #include <new>
 
 #include "rbf_assembly.h"
-#include "rbf_operators.h"
+#include "rbf_wave.h"
 #include "rbf_mixed.h"
 #include "rbf_nullspace.h"
 #include "assemble_common.h"

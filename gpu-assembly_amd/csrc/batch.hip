@@ -24,7 +24,7 @@
 #include "assemble_common.h"
 #include "rbf_assembly.h"
 #include "rbf_group.h"
-#include "rbf_operators.h"
+#include "rbf_wave.h"
 #include "rbf_tile.h"
 
 namespace {

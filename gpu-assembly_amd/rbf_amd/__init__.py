@@ -108,6 +108,8 @@ def lib():
         "rbfa_read_points": (i32, [ctypes.c_char_p, i32, P, P, P]),
         "rbfa_read_graph": (i32, [ctypes.c_char_p, i32, P, P, P, P, i64]),
         "rbfa_write_matrix_market": (i32, [ctypes.c_char_p, i64, i64, P, P, P, i32]),
+        "rbfa_gesv_batched": (i32, [i32, i32, i32, P, P, P, P, P]),
+        "rbfa_solve_dp": (i32, [P, P, P, P, P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -517,6 +519,25 @@ def measure_peaks(stream=None):
     out = (ctypes.c_double * 3)()
     _check(lib().rbfa_measure_peaks(out, _stream(stream)), "rbfa_measure_peaks")
     return {"fp64_valu_tflops": out[0], "fp64_mfma_tflops": out[1], "hbm_copy_gbs": out[2]}
+
+
+def gesv_batched(A, B=None, ipiv=None, info=None, stream=None):
+    """Solve a batch of caller-filled systems in place (the reference's
+    execute(A, ipiv, B, info), rbf_operators.h:29-37). A: float64 CUDA
+    tensor (nbatch, nt, nt) holding each system column-major (A[b, j, i] is
+    row i, column j); B: (nbatch, nrhs, nt), column-major likewise, or None.
+    On return A holds the dgetrf-layout LU factors, B the solution; returns
+    (ipiv (nbatch, nt) int32, 1-based; info (nbatch,) int32)."""
+    import torch
+    nbatch, nt = int(A.shape[0]), int(A.shape[1])
+    nrhs = 0 if B is None else int(B.shape[1])
+    if ipiv is None:
+        ipiv = torch.empty((nbatch, nt), dtype=torch.int32, device=A.device)
+    if info is None:
+        info = torch.empty(nbatch, dtype=torch.int32, device=A.device)
+    _check(lib().rbfa_gesv_batched(nt, nrhs, nbatch, _dptr(A), _dptr(ipiv), _dptr(B),
+                                   _dptr(info), _stream(stream)), "rbfa_gesv_batched")
+    return ipiv, info
 
 
 def apply(ja, w, u, y, k=None, ia=None, bnd=None, stream=None):

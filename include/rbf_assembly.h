@@ -282,6 +282,28 @@ RBFA_API int32_t rbfa_batch_get_mode(const rbfa_batch* batch);
 RBFA_API int32_t rbfa_batch_launch(rbfa_batch* batch, void* stream);
 RBFA_API int32_t rbfa_batch_autotune(rbfa_batch* batch, void* stream, int32_t reps, float* ms_out);
 
+/* ------------------------------------------------------------ caller-filled systems
+ * The reference's solve of a system its caller filled: execute(A, ipiv, B,
+ * info) of gesv_partial_pivot (rbf_operators.h:29-37), exported to Fortran
+ * as extern "C" solve_dp (rbf_operators.cpp:7-11, a 15 x 15 system with
+ * NRHS = 9, launched from main.f90:57). Device pointers, stream-ordered.
+ *
+ * rbfa_gesv_batched: nbatch independent systems, one after another in
+ * memory: system b is A + b*nt*nt (column-major nt x nt, leading dimension
+ * nt), right-hand sides B + b*nt*nrhs (column-major nt x nrhs), pivots
+ * ipiv + b*nt, status info[b] (info may be NULL). 1 <= nt <= 64,
+ * 0 <= nrhs <= 16 (0: factor only, B may be NULL). On return A holds the
+ * LU factors in LAPACK dgetrf layout (L unit lower, U upper, the row that
+ * was pivot at step k in row k), ipiv the 1-based dgetrf interchanges, B
+ * the solution, info[b] 0 or the 1-based index of the first exactly zero
+ * pivot. The pivot is the first largest |a| in row order; the factors and
+ * the solution are bit-identical to the textbook FP64 elimination.
+ *
+ * rbfa_solve_dp: one 15 x 15 system with 9 right-hand sides (solve_dp). */
+RBFA_API int32_t rbfa_gesv_batched(int32_t nt, int32_t nrhs, int32_t nbatch, double* A,
+                                   int32_t* ipiv, double* B, int32_t* info, void* stream);
+RBFA_API int32_t rbfa_solve_dp(double* A, int32_t* ipiv, double* B, int32_t* info, void* stream);
+
 /* ------------------------------------------------------------ operator application
  * Replaces the mv_op of src/main.cpp:59-66 (A.spmv(x, y), then
  * y[bnd[k]] = x[bnd[k]]): y_c = W_c u for nops (1..4) assembled operators

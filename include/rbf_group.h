@@ -14,7 +14,7 @@
 #ifndef RBF_GROUP_H
 #define RBF_GROUP_H
 
-#include "rbf_operators.h"
+#include "rbf_wave.h"
 
 namespace rbf_operators {
 namespace detail {

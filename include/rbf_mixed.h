@@ -20,7 +20,7 @@
 #ifndef RBF_MIXED_H
 #define RBF_MIXED_H
 
-#include "rbf_operators.h"
+#include "rbf_wave.h"
 
 namespace rbf_operators {
 namespace detail {

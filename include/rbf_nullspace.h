@@ -41,7 +41,7 @@
 #ifndef RBF_NULLSPACE_H
 #define RBF_NULLSPACE_H
 
-#include "rbf_operators.h"
+#include "rbf_wave.h"
 
 namespace rbf_operators {
 namespace detail {

@@ -129,18 +129,19 @@ struct TileAssembly {
 
     // Gather, shift and scale stencil s into LDS (points, monomials), as
     // GroupAssembly::gather does; reset the pivot bookkeeping. Returns rho^2.
+    template <typename Coord>
     __device__ __forceinline__ static double gather(const int s, const size_t base,
-                                                    const double* __restrict__ x,
-                                                    const double* __restrict__ y,
-                                                    const double* __restrict__ z,
+                                                    const Coord* __restrict__ x,
+                                                    const Coord* __restrict__ y,
+                                                    nodeduce<const Coord*> __restrict__ z,
                                                     const int* __restrict__ ja, double* lds, int g,
                                                     int w) {
 #pragma clang fp contract(off)
         const int lane = lane_id();
         const int jn = g < N ? ja[base + g] : s;
-        const double px = x[jn] - x[s];
-        const double py = y[jn] - y[s];
-        const double pz = DIM == 3 ? z[jn] - z[s] : 0.0;
+        const double px = static_cast<double>(x[jn]) - static_cast<double>(x[s]);
+        const double py = static_cast<double>(y[jn]) - static_cast<double>(y[s]);
+        const double pz = DIM == 3 ? static_cast<double>(z[jn]) - static_cast<double>(z[s]) : 0.0;
         double d2 = px * px + py * py;
         if constexpr (DIM == 3) d2 = d2 + pz * pz;
         const double wm = wave_max(g < N ? d2 : 0.0);
@@ -495,18 +496,20 @@ struct TileAssembly {
     }
 
     // Stencil s, whose CSR row starts at base (s*N for a fixed-width graph).
-    template <class Out>
+    // Coordinates of type Coord (double or float) are widened on load;
+    // out(c) points at Coord.
+    template <class Out, typename Coord>
     __device__ __forceinline__ static void run(const int s, const size_t base,
-                                               const double* __restrict__ x,
-                                               const double* __restrict__ y,
-                                               const double* __restrict__ z,
+                                               const Coord* __restrict__ x,
+                                               const Coord* __restrict__ y,
+                                               nodeduce<const Coord*> __restrict__ z,
                                                const int* __restrict__ ja, Out out, double* lds,
                                                int* __restrict__ info) {
 #pragma clang fp contract(off)
         const int w = static_cast<int>(threadIdx.x >> 6);
         const int lane = lane_id();
         const int g = w * 64 + lane;
-        const double rho2 = gather(s, base, x, y, z, ja, lds, g, w);
+        const double rho2 = gather<Coord>(s, base, x, y, z, ja, lds, g, w);
         const double rho = sqrt(rho2);
         dbl4 tile[LC][RT];
         {
@@ -559,7 +562,7 @@ struct TileAssembly {
 #pragma unroll
                         for (int c = 0; c < R; ++c) {
                             const double v = (set ? b1[c] : b0[c]) * myrcp;
-                            out(c)[at] = c == 0 ? v / rho2 : v / rho;
+                            out(c)[at] = static_cast<Coord>(c == 0 ? v / rho2 : v / rho);
                         }
                     }
                 }

@@ -44,6 +44,8 @@ def lib():
         L.oracle_rhs.argtypes = [ctypes.c_int] * 6 + [P] * 7
         L.oracle_gesv.restype = ctypes.c_int
         L.oracle_gesv.argtypes = [ctypes.c_int, ctypes.c_int, P, P, P]
+        L.oracle_getrf.restype = ctypes.c_int
+        L.oracle_getrf.argtypes = [ctypes.c_int, P, P, ctypes.POINTER(ctypes.c_int)]
         L.oracle_assemble.restype = ctypes.c_int
         L.oracle_assemble.argtypes = [ctypes.c_int] * 6 + [i32, i32] + [P] * 7 + [P, P, ctypes.c_int]
         L.oracle_assemble_rows.restype = ctypes.c_int
@@ -118,6 +120,18 @@ def gesv(M, B):
     X = np.zeros((nt, r), order="F")
     info = lib().oracle_gesv(nt, r, _p(M.ravel(order="F")), _p(B.ravel(order="F")), _p(X))
     return X, info
+
+
+def getrf(M):
+    """LU factors (dgetrf layout, column-major nt x nt), 1-based pivots and
+    info of the oracle's elimination."""
+    M = np.asfortranarray(M, float)
+    nt = M.shape[0]
+    LU = np.zeros((nt, nt), order="F")
+    ipiv = np.zeros(nt, np.int32)
+    info = lib().oracle_getrf(nt, _p(M.ravel(order="F")), _p(LU),
+                              ipiv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return LU, ipiv, info
 
 
 def assemble(dim, n, P, Q, x, y, ja, z=None, op_set=0, centres=None, s0=0, count=None,

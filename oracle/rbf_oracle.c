@@ -15,6 +15,12 @@
  *                  point is sketched at main.cpp:15-16 and packed as
  *                  W(:,1)+W(:,2) at rbf_cuda.f90:139-157)
  *   gather/scatter assemble_interp_weights       rbf_operators.h:340-375
+ *   oracle_getrf() the factors and pivots that GESV leaves in A and ipiv
+ *                  (execute(A, ipiv, B, info), rbf_operators.h:29-37): the
+ *                  same elimination as oracle_gesv, written out in LAPACK
+ *                  dgetrf layout (row k of the output = the row that was
+ *                  pivot at step k; 1-based sequential-swap ipiv); pinned
+ *                  against scipy's dgetrf (tests/test_oracle.py)
  *   oracle_gesv()  the block GESV of cuSolverDx (rbf_operators.h:365),
  *                  closed and absent: restated as textbook partial-pivot
  *                  LU (Golub & Van Loan Alg. 3.4.1) -- parity unpinned
@@ -195,6 +201,66 @@ OR_API void oracle_rhs(int dim, int n, int P, int Q, int op_set, int nrhs, const
  * and B nt x r, both column-major; X (nt x r, column-major) receives the
  * solution. work holds nt*(nt+r) doubles + 3*nt ints' worth of space.
  * Returns info. */
+/* LU factors and pivots in dgetrf layout, from the elimination of
+ * oracle_gesv (rows never moved, first largest |a| among rows not yet
+ * pivotal). Row p_k, pivot at step k, becomes row k of LU: its multipliers
+ * of steps j < k (stored when it was eliminated) and its U entries j >= k.
+ * ipiv[k] (1-based) is the position, in the order left by the swaps of
+ * steps 0..k-1, of row p_k -- dgetrf's sequential row interchanges. */
+OR_API int oracle_getrf(int nt, const double* M, double* LU, int* ipiv) {
+    double* a = (double*)malloc(sizeof(double) * (size_t)nt * nt);
+    int* done = (int*)malloc(sizeof(int) * 3 * (size_t)nt);
+    int* step = done + nt;  /* step at which each row was pivot */
+    int* pos = step + nt;   /* current position of each row */
+    int info = 0;
+    for (int i = 0; i < nt; ++i) {
+        for (int j = 0; j < nt; ++j) a[i * nt + j] = M[i + j * nt];
+        done[i] = 0;
+        pos[i] = i;
+    }
+    for (int k = 0; k < nt; ++k) {
+        double best = -1.0;
+        int p = -1;
+        for (int i = 0; i < nt; ++i) {
+            if (done[i]) continue;
+            const double v = fabs(a[i * nt + k]);
+            if (v > best) {
+                best = v;
+                p = i;
+            }
+        }
+        if (p < 0) {
+            for (int i = 0; i < nt; ++i)
+                if (!done[i]) { p = i; break; }
+            if (!info) info = k + 1;
+        }
+        const double piv = a[p * nt + k];
+        if (piv == 0.0 && !info) info = k + 1;
+        const double rcp = 1.0 / piv;
+        /* the swap of positions k and pos[p] */
+        const int q = pos[p];
+        ipiv[k] = q + 1;
+        for (int i = 0; i < nt; ++i)
+            if (pos[i] == k) pos[i] = q;
+        pos[p] = k;
+        done[p] = 1;
+        step[p] = k;
+        const double* up = &a[p * nt];
+        for (int i = 0; i < nt; ++i) {
+            if (done[i]) continue;
+            double* ai = &a[i * nt];
+            const double l = ai[k] * rcp;
+            ai[k] = l;
+            for (int j = k + 1; j < nt; ++j) ai[j] = fma(-l, up[j], ai[j]);
+        }
+    }
+    for (int i = 0; i < nt; ++i)
+        for (int j = 0; j < nt; ++j) LU[step[i] + j * nt] = a[i * nt + j];
+    free(a);
+    free(done);
+    return info;
+}
+
 OR_API int oracle_gesv(int nt, int r, const double* M, const double* B, double* X) {
     const int nc = nt + r;
     double* aug = (double*)malloc(sizeof(double) * (size_t)nt * nc + sizeof(double) * nt);

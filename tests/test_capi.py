@@ -91,6 +91,21 @@ def test_device_node_layer_argument_checks():
     assert L.rbfa_gather_f64_device(10, 1, 2, 2, None) == R.ERR_ARG
 
 
+def test_gesv_argument_checks_before_any_launch():
+    L = R.lib()
+    d = ctypes.c_void_p(16)  # never dereferenced: every call below fails its checks
+    assert L.rbfa_gesv_batched(0, 1, 1, d, d, d, d, None) == R.ERR_ARG
+    assert L.rbfa_gesv_batched(65, 1, 1, d, d, d, d, None) == R.ERR_ARG
+    assert L.rbfa_gesv_batched(15, 17, 1, d, d, d, d, None) == R.ERR_ARG
+    assert L.rbfa_gesv_batched(15, -1, 1, d, d, d, d, None) == R.ERR_ARG
+    assert L.rbfa_gesv_batched(15, 9, -1, d, d, d, d, None) == R.ERR_ARG
+    assert L.rbfa_gesv_batched(15, 9, 1, None, d, d, d, None) == R.ERR_ARG
+    assert L.rbfa_gesv_batched(15, 9, 1, d, None, d, d, None) == R.ERR_ARG
+    assert L.rbfa_gesv_batched(15, 9, 1, d, d, None, d, None) == R.ERR_ARG
+    assert L.rbfa_gesv_batched(15, 9, 0, None, None, None, None, None) == R.OK
+    assert L.rbfa_solve_dp(d, d, d, None, None) == R.ERR_ARG
+
+
 def test_assemble_rejects_null_plan():
     assert R.lib().rbfa_assemble(None, 10, None, None, None, None, None, None, None, None, None,
                                  None, None) == R.ERR_ARG

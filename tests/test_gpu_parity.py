@@ -309,3 +309,81 @@ def test_large_systems_mixed(dim, m, n, P, Q):
         assert e[fb].max() <= FP64_TOL
     # the default variants converge: FP64 fallback stays the exception
     assert fb.mean() <= 0.05
+
+
+# ---------------------------------------------------------------- caller-filled systems
+
+def _demo_system():
+    """The reference demo's system (main.f90:22-57): the 9-point stencil
+    x, y in {-1, 0, 1}, PHS r^3 + the 6 quadratic monomials (15 x 15,
+    rbf_cuda.f90:36-99), interpolation to the 9 centres (0.5 x, 0.5 y)
+    (fill_rhs, rbf_cuda.f90:104-136)."""
+    x = np.array([0, 1, 0, -1, 0, 1, 1, -1, -1], float)
+    y = np.array([0, 0, 1, 0, -1, 1, -1, -1, 1], float)
+    M = O.fill(2, 9, 2, 3, x, y)
+    B = O.rhs(2, 9, 2, 3, x, y, op_set=1, centres=(0.5 * x, 0.5 * y))
+    return M, B
+
+
+@pytest.mark.parametrize("nt,nrhs,nbatch", [(15, 9, 1), (15, 9, 333), (27, 3, 200), (45, 3, 129),
+                                             (64, 16, 65), (7, 0, 10), (33, 5, 17)])
+def test_gesv_batched_matches_oracle(nt, nrhs, nbatch):
+    """Factors (dgetrf layout), pivots and solution bit-identical to the
+    oracle's elimination; random systems plus the reference demo's system."""
+    import torch
+    rng = np.random.default_rng(nt * 1000 + nrhs)
+    Ms = rng.standard_normal((nbatch, nt, nt))
+    Bs = rng.standard_normal((nbatch, nt, nrhs))
+    if nt == 15 and nrhs == 9:
+        Ms[0], Bs[0] = _demo_system()
+    A = torch.from_numpy(np.ascontiguousarray(Ms.transpose(0, 2, 1))).to("cuda:0")
+    B = torch.from_numpy(np.ascontiguousarray(Bs.transpose(0, 2, 1))).to("cuda:0") if nrhs else None
+    ipiv, info = R.gesv_batched(A, B)
+    torch.cuda.synchronize()
+    LU = A.cpu().numpy().transpose(0, 2, 1)
+    X = B.cpu().numpy().transpose(0, 2, 1) if nrhs else None
+    ipiv, info = ipiv.cpu().numpy(), info.cpu().numpy()
+    for b in range(nbatch):
+        LUr, pr, ir = O.getrf(Ms[b])
+        np.testing.assert_array_equal(LU[b], LUr)
+        np.testing.assert_array_equal(ipiv[b], pr)
+        assert info[b] == ir == 0
+        if nrhs:
+            Xr, _ = O.gesv(Ms[b], Bs[b])
+            np.testing.assert_array_equal(X[b], Xr)
+
+
+def test_gesv_batched_singular_and_args():
+    import torch
+    M = np.array([[1.0, 2.0, 0.0], [2.0, 4.0, 0.0], [0.0, 0.0, 1.0]])
+    A = torch.from_numpy(np.ascontiguousarray(M.T[None])).to("cuda:0")
+    B = torch.ones((1, 1, 3), dtype=torch.float64, device="cuda:0")
+    ipiv, info = R.gesv_batched(A, B)
+    torch.cuda.synchronize()
+    assert int(info[0]) == O.gesv(M, np.ones((3, 1)))[1] == 2
+    LUr, pr, ir = O.getrf(M)
+    np.testing.assert_array_equal(A.cpu().numpy()[0].T, LUr)
+    np.testing.assert_array_equal(ipiv.cpu().numpy()[0], pr)
+    big = torch.zeros((1, 65, 65), dtype=torch.float64, device="cuda:0")
+    with pytest.raises(R.RbfError):
+        R.gesv_batched(big)
+
+
+def test_solve_dp_demo_system():
+    """rbfa_solve_dp: the reference's extern "C" solve_dp (15 x 15, NRHS 9)."""
+    import ctypes
+    import torch
+    M, B = _demo_system()
+    A = torch.from_numpy(np.ascontiguousarray(M.T)).to("cuda:0")
+    Bd = torch.from_numpy(np.ascontiguousarray(B.T)).to("cuda:0")
+    ipiv = torch.zeros(15, dtype=torch.int32, device="cuda:0")
+    info = torch.full((1,), -1, dtype=torch.int32, device="cuda:0")
+    st = R.lib().rbfa_solve_dp(ctypes.c_void_p(A.data_ptr()), ctypes.c_void_p(ipiv.data_ptr()),
+                               ctypes.c_void_p(Bd.data_ptr()), ctypes.c_void_p(info.data_ptr()),
+                               ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+    assert st == 0 and int(info[0]) == 0
+    Xr, _ = O.gesv(M, B)
+    np.testing.assert_array_equal(Bd.cpu().numpy().T, Xr)
+    # interpolation at the stencil's own centre (0, 0) reproduces node 0
+    np.testing.assert_allclose(Xr[:9, 0], np.eye(9)[0], atol=1e-12)

@@ -183,6 +183,33 @@ def test_gesv_partial_pivoting_handles_zero_diagonal():
     np.testing.assert_array_equal(X[:, 0], [3.0, 2.0])
 
 
+@pytest.mark.parametrize("nt", [5, 15, 45, 64])
+def test_getrf_layout_matches_lapack(nt):
+    """The factors and pivots GESV leaves in A and ipiv, in dgetrf's layout:
+    pivots equal scipy's dgetrf (no ties in a random matrix), factors agree
+    to rounding, and P A = L U holds with the 1-based sequential swaps."""
+    import scipy.linalg as sl
+    A = np.random.default_rng(nt).standard_normal((nt, nt))
+    LU, ipiv, info = O.getrf(A)
+    lu, piv = sl.lu_factor(A)
+    assert info == 0
+    np.testing.assert_array_equal(ipiv - 1, piv)
+    assert np.abs(LU - lu).max() <= 1e-12 * np.abs(lu).max()
+    L = np.tril(LU, -1) + np.eye(nt)
+    U = np.triu(LU)
+    PA = A.copy()
+    for k, q in enumerate(ipiv - 1):
+        PA[[k, q]] = PA[[q, k]]
+    assert np.abs(PA - L @ U).max() <= 64 * nt * np.finfo(float).eps * np.abs(A).max()
+    # a transposed operand gives other pivots
+    _, ipiv_t, _ = O.getrf(A.T)
+    assert (ipiv_t != ipiv).any()
+    # the same elimination as the solve: X from the factors equals oracle_gesv's
+    B = np.random.default_rng(1).standard_normal((nt, 3))
+    X, _ = O.gesv(A, B)
+    np.testing.assert_allclose(sl.lu_solve((LU, ipiv - 1), B), X, rtol=0, atol=1e-10 * np.abs(X).max())
+
+
 def test_gesv_singular_info():
     M = np.array([[1.0, 2.0, 0.0], [2.0, 4.0, 0.0], [0.0, 0.0, 1.0]])
     _, info = O.gesv(M, np.ones((3, 1)))
