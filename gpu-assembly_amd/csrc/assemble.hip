@@ -8,6 +8,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstring>
 #include <new>
@@ -257,6 +258,68 @@ struct rbfa_plan {
     int64_t ws_cap = 0;
 };
 
+namespace {
+
+// The fallback workspace of rbfa_assemble_noalloc: [0] the count of
+// stencils listed for the FP64 pass (cleared by a fill kernel at the start
+// of every launch), [1 ..] the list. One copy per device (16 MiB), held
+// until the process ends; every call on a device shares it.
+constexpr int32_t kNoAllocMaxStencils = 1 << 22;
+static __device__ __align__(256) int scratch[kNoAllocMaxStencils + 1];
+
+constexpr int kMaxDevices = 64;
+std::atomic<int*> g_scratch_addr[kMaxDevices];
+
+// Device address of the current device's copy of scratch. The name itself
+// is a host-side stand-in: only the address hipGetSymbolAddress returns,
+// with the device current, may reach a kernel. Looked up once per device.
+int32_t scratch_of_current_device(int** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return RBFA_ERR_NODEVICE;
+    if (dev < 0 || dev >= kMaxDevices) return RBFA_ERR_UNSUPPORTED;
+    int* p = g_scratch_addr[dev].load(std::memory_order_acquire);
+    if (!p) {
+        void* a = nullptr;
+        if (hipGetSymbolAddress(&a, HIP_SYMBOL(scratch)) != hipSuccess || !a) return RBFA_ERR_HIP;
+        p = static_cast<int*>(a);
+        g_scratch_addr[dev].store(p, std::memory_order_release);
+    }
+    *out = p;
+    return RBFA_OK;
+}
+
+// Checks the arguments of rbfa_assemble and fills the launch arguments
+// (all but the workspace).
+int32_t prepare(const rbfa_plan* plan, int32_t nstencils, const double* x, const double* y,
+                const double* z, const int32_t* ja, double* const* w, const double* xc,
+                const double* yc, const double* zc, int32_t* info, uint8_t* status, void* stream,
+                LaunchArgs& a) {
+    const Shape& sh = plan->shape;
+    if (!x || !y || !ja || !w || (sh.dim == 3 && !z)) return RBFA_ERR_ARG;
+    if (static_cast<int64_t>(nstencils) * sh.n > INT32_MAX) return RBFA_ERR_ARG;
+    if (sh.op_set == RBFA_OPS_INTERP && (!xc || !yc || (sh.dim == 3 && !zc))) return RBFA_ERR_ARG;
+    a = LaunchArgs{};
+    a.nstencils = nstencils;
+    a.x = x;
+    a.y = y;
+    a.z = z;
+    a.ja = ja;
+    for (int c = 0; c < sh.nops; ++c) {
+        if (!w[c]) return RBFA_ERR_ARG;
+        a.w[c] = w[c];
+    }
+    a.cx = xc;
+    a.cy = yc;
+    a.cz = zc;
+    a.info = info;
+    a.status = status;
+    a.max_refine = plan->desc.max_refine;
+    a.stream = static_cast<hipStream_t>(stream);
+    return RBFA_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 RBFA_API const char* rbfa_version(void) { return "rbf_amd 0.1 (gfx950)"; }
@@ -307,26 +370,9 @@ RBFA_API int32_t rbfa_assemble(const rbfa_plan* plan, int32_t nstencils, const d
     if (!plan || nstencils < 0) return RBFA_ERR_ARG;
     if (nstencils == 0) return RBFA_OK;
     const Shape& sh = plan->shape;
-    if (!x || !y || !ja || !w || (sh.dim == 3 && !z)) return RBFA_ERR_ARG;
-    if (static_cast<int64_t>(nstencils) * sh.n > INT32_MAX) return RBFA_ERR_ARG;
-    if (sh.op_set == RBFA_OPS_INTERP && (!xc || !yc || (sh.dim == 3 && !zc))) return RBFA_ERR_ARG;
-    LaunchArgs a{};
-    a.nstencils = nstencils;
-    a.x = x;
-    a.y = y;
-    a.z = z;
-    a.ja = ja;
-    for (int c = 0; c < sh.nops; ++c) {
-        if (!w[c]) return RBFA_ERR_ARG;
-        a.w[c] = w[c];
-    }
-    a.cx = xc;
-    a.cy = yc;
-    a.cz = zc;
-    a.info = info;
-    a.status = status;
-    a.max_refine = plan->desc.max_refine;
-    a.stream = static_cast<hipStream_t>(stream);
+    LaunchArgs a;
+    const int32_t rc = prepare(plan, nstencils, x, y, z, ja, w, xc, yc, zc, info, status, stream, a);
+    if (rc != RBFA_OK) return rc;
     if (sh.needs_ws) {
         // fallback count + list of up to nstencils entries; grown on demand
         auto* pl = const_cast<rbfa_plan*>(plan);
@@ -344,6 +390,27 @@ RBFA_API int32_t rbfa_assemble(const rbfa_plan* plan, int32_t nstencils, const d
         a.ws = pl->ws;
     }
     return sh.variants[plan->variant].fn(a) == hipSuccess ? RBFA_OK : RBFA_ERR_HIP;
+}
+
+RBFA_API int32_t rbfa_assemble_noalloc(const rbfa_plan* plan, int32_t nstencils, const double* x,
+                                       const double* y, const double* z, const int32_t* ja,
+                                       double* const* w, const double* xc, const double* yc,
+                                       const double* zc, int32_t* info, uint8_t* status,
+                                       void* stream) {
+    if (!plan || nstencils < 0) return RBFA_ERR_ARG;
+    if (nstencils == 0) return RBFA_OK;
+    if (nstencils > kNoAllocMaxStencils) {
+        set_last_error("rbfa_assemble_noalloc: more than 4194304 stencils in one call");
+        return RBFA_ERR_ARG;
+    }
+    LaunchArgs a;
+    int32_t rc = prepare(plan, nstencils, x, y, z, ja, w, xc, yc, zc, info, status, stream, a);
+    if (rc != RBFA_OK) return rc;
+    if (plan->shape.needs_ws) {
+        rc = scratch_of_current_device(&a.ws);
+        if (rc != RBFA_OK) return rc;
+    }
+    return plan->shape.variants[plan->variant].fn(a) == hipSuccess ? RBFA_OK : RBFA_ERR_HIP;
 }
 
 RBFA_API int32_t rbfa_plan_num_variants(const rbfa_plan* plan) { return plan ? plan->shape.nvariants : -1; }

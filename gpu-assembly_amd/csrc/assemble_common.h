@@ -24,6 +24,9 @@ struct LaunchArgs {
     hipStream_t stream;
 };
 
+// Sets the calling thread's rbfa_last_error text (host_io.cpp).
+void set_last_error(const char* what);
+
 // MI355X dispatches consecutive workgroups round-robin over its 8 XCDs,
 // each with its own L2. Renumber so each XCD works on one contiguous range
 // of (space-filling-curve ordered) stencils and its L2 sees neighbouring

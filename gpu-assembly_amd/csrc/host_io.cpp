@@ -54,6 +54,11 @@ struct Cursor {
 
 }  // namespace
 
+namespace rbf_amd_internal {
+// declared for the kernel translation units in assemble_common.h
+void set_last_error(const char* what) { g_last_error = what; }
+}  // namespace rbf_amd_internal
+
 extern "C" {
 
 RBFA_API const char* rbfa_last_error(void) { return g_last_error.c_str(); }

@@ -80,6 +80,7 @@ def lib():
         "rbfa_plan_nops": (i32, [P]),
         "rbfa_plan_system_size": (i32, [P]),
         "rbfa_assemble": (i32, [P, i32, P, P, P, P, P, P, P, P, P, P, P]),
+        "rbfa_assemble_noalloc": (i32, [P, i32, P, P, P, P, P, P, P, P, P, P, P]),
         "rbfa_plan_num_variants": (i32, [P]),
         "rbfa_plan_variant_name": (ctypes.c_char_p, [P, i32]),
         "rbfa_plan_get_variant": (i32, [P]),
@@ -677,11 +678,14 @@ class Plan:
         return list(ms)
 
     def assemble(self, nstencils, x, y, ja, w, z=None, centres=None, info=None, status=None,
-                 stream=None):
+                 stream=None, noalloc=False):
         """Enqueue the assembly. x, y, z, ja, w[*], info, status are torch CUDA
-        tensors (float64 / int32 / uint8); centres = (xc, yc[, zc]) tensors."""
+        tensors (float64 / int32 / uint8); centres = (xc, yc[, zc]) tensors.
+        noalloc: call rbfa_assemble_noalloc (no device allocation; at most
+        4194304 stencils; calls on one device must not overlap)."""
         args = self._args(nstencils, x, y, ja, w, z, centres, info, status, stream)
-        _check(lib().rbfa_assemble(self._h, *args), "rbfa_assemble")
+        fn = "rbfa_assemble_noalloc" if noalloc else "rbfa_assemble"
+        _check(getattr(lib(), fn)(self._h, *args), fn)
 
     def _args(self, nstencils, x, y, ja, w, z, centres, info, status, stream):
         import torch

@@ -228,6 +228,23 @@ RBFA_API int32_t rbfa_assemble(const rbfa_plan* plan, int32_t nstencils, const d
                                double* const* w, const double* xc, const double* yc,
                                const double* zc, int32_t* info, uint8_t* status, void* stream);
 
+/* rbfa_assemble without any allocation: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, checks and results as rbfa_assemble; the mixed-form shapes
+ * whose stencils may be listed for an FP64 pass (2-D n = 56, 3-D n = 80)
+ * keep that list in a fixed per-device buffer inside the library (16 MiB,
+ * held until the process ends) instead of the plan's own workspace.
+ * Limit: nstencils <= 4194304 (1 << 22) per call; larger calls return
+ * RBFA_ERR_ARG and rbfa_last_error says why. Calls that overlap on one
+ * device (other streams, other host threads) share that buffer and race:
+ * order them on one stream. Call with the plan's device current. */
+RBFA_API int32_t rbfa_assemble_noalloc(const rbfa_plan* plan, int32_t nstencils, const double* x,
+                                       const double* y, const double* z, const int32_t* ja,
+                                       double* const* w, const double* xc, const double* yc,
+                                       const double* zc, int32_t* info, uint8_t* status,
+                                       void* stream);
+
 /* ---- kernel variants and the autotuner --------------------------------
  * Each shape ships several kernel variants (pivot-row broadcast through
  * SGPRs or LDS, pivot search by shuffles or DPP, waves per workgroup). All
