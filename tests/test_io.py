@@ -53,31 +53,71 @@ def test_reads_reference_data_files(stem, k):
 
 
 @pytest.mark.parametrize("text,k,what", [
-    ("3\n", 0, "malformed header"),
-    ("2 3\n0 1\n\n", 0, "has no entries"),
-    ("2 3\n0 1\n2 1\n", 0, "outside [0, 2)"),
-    ("2 3\n0 0\n1\n", 0, "listed twice"),
-    ("2 4\n0 1\n1\n", 2, "expected k=2"),
-    ("2 3\n0 1\n1\n1 0\n", 0, "unexpected data after row 1"),
-    ("2 5\n0 1\n1\n", 0, "entries, found 3"),
-    ("2 3\n0 1\n1 x\n", 0, "bad index"),
-    ("2 3\n0 1.5\n1 0\n", 0, "bad index"),
-    ("3 4\n0 1\n1\n", 0, "found 2"),
+    ("3\n", 0, "line 1 must hold two counts"),
+    ("2 3 4\n0 1\n1\n", 0, "line 1 must hold two counts"),
+    ("2 3\n0 1\n\n", 0, "stencil row 1 is empty"),
+    ("2 3\n0 1\n2 1\n", 0, "node index 2 lies outside 0..1"),
+    ("2 3\n0 1\n-1 1\n", 0, "node index -1 lies outside 0..1"),
+    ("2 3\n0 0\n1\n", 0, "stencil row 0 names node 0 twice"),
+    ("2 4\n0 1\n1\n", 2, "stencil row 1 holds 1 nodes; fixed-size stencils hold 2"),
+    ("2 5\n0 1\n1\n", 2, "fixed size k = 2 need 4 entries"),
+    ("2 3\n0 1\n1\n1 0\n", 0, "text follows the last of the 2 stencil rows"),
+    ("2 5\n0 1\n1\n", 0, "the rows hold 3 entries; line 1 declares 5"),
+    ("2 2\n0 1\n1\n", 0, "more than the 2 entries"),
+    ("2 3\n0 1\n1 x\n", 0, "'x' is not an integer node index"),
+    ("2 3\n0 1.5\n1 0\n", 0, "'1.5' is not an integer node index"),
+    ("3 4\n0 1\n1\n", 0, "the file ends after 2 of 3 stencil rows"),
 ])
-def test_graph_validation_follows_the_reference(tmp_path, text, k, what):
-    # the checks of read_graph_csr (rbf_io.h:630-707)
+def test_graph_validation_follows_the_format(tmp_path, text, k, what):
+    # the rules of the graph file (rbf_io.h:630-648, docs/file_formats.md)
     p = tmp_path / "bad.graph"
     p.write_text(text)
     with pytest.raises(R.RbfError) as e:
         R.read_graph(str(p), k)
     assert e.value.status == R.ERR_FORMAT
-    assert what in str(e.value), str(e.value)
+    assert what in R.lib().rbfa_last_error().decode(), R.lib().rbfa_last_error()
+
+
+def test_graph_reader_accepts_crlf_and_trailing_blank_lines(tmp_path):
+    p = tmp_path / "ok.graph"
+    p.write_bytes(b"3 6\r\n0 1\r\n1 2 0\r\n2\r\n\r\n  \n")
+    ia, ja = R.read_graph(str(p))
+    np.testing.assert_array_equal(ia, [0, 2, 5, 6])
+    np.testing.assert_array_equal(ja, [0, 1, 1, 2, 0, 2])
+
+
+@pytest.mark.parametrize("text,what", [
+    ("", "line 1 must hold the node count"),
+    ("2 3\n0 0\n1 1\n", "line 1 must hold the node count"),
+    ("x\n", "line 1 must hold the node count"),
+    ("3\n0 0\n1 1\n", "the file ends after 2 of 3 coordinate pairs"),
+    ("2\n0 0\n1\n", "line 3: a node takes exactly two coordinates"),
+    ("2\n0 0 0\n1 1\n", "line 2: a node takes exactly two coordinates"),
+    ("2\n0 0\n1 nan\n", "line 3: coordinates must be finite"),
+    ("2\n0 inf\n1 1\n", "line 2: coordinates must be finite"),
+])
+def test_points_validation_follows_the_format(tmp_path, text, what):
+    # the rules of the points file (rbf_io.h:198-221, docs/file_formats.md)
+    p = tmp_path / "bad.points"
+    p.write_text(text)
+    with pytest.raises(R.RbfError) as e:
+        R.read_points(str(p))
+    assert e.value.status == R.ERR_FORMAT
+    assert what in R.lib().rbfa_last_error().decode(), R.lib().rbfa_last_error()
+
+
+def test_points_after_the_last_pair_are_ignored(tmp_path):
+    p = tmp_path / "ok.points"
+    p.write_text("2\n0.5 -1e-3\n2 3\nanything at all\n")
+    x, y = R.read_points(str(p))
+    np.testing.assert_array_equal(x, [0.5, 2.0])
+    np.testing.assert_array_equal(y, [-1e-3, 3.0])
 
 
 def test_missing_file_is_an_io_error(tmp_path):
     with pytest.raises(R.RbfError) as e:
         R.read_points(str(tmp_path / "none.points"))
-    assert e.value.status == R.ERR_IO and "cannot open" in str(e.value)
+    assert e.value.status == R.ERR_IO and "cannot open" in R.lib().rbfa_last_error().decode()
 
 
 def test_matrix_market_matches_the_reference_writer(tmp_path):
