@@ -360,7 +360,12 @@ RBFA_API int32_t rbfa_batch_bind(rbfa_batch* b, int32_t nstencils, const int32_t
                                  const double* x, const double* y, const double* z,
                                  const int32_t* ia, const int32_t* ja, double* const* w,
                                  int32_t* info, uint8_t* status, void* stream) {
-    if (!b || nstencils < 0 || (nstencils > 0 && (!ia_host || !x || !y || !ia || !ja || !w)))
+    if (!b) return RBFA_ERR_ARG;
+    // unbound until every step below has succeeded: a failed bind leaves a
+    // batch that launch and autotune refuse, never one that mixes the old
+    // binding with parts of the new
+    b->bound = false;
+    if (nstencils < 0 || (nstencils > 0 && (!ia_host || !x || !y || !ia || !ja || !w)))
         return RBFA_ERR_ARG;
     for (int c = 0; c < 3; ++c)
         if (nstencils > 0 && !w[c]) return RBFA_ERR_ARG;
@@ -376,9 +381,10 @@ RBFA_API int32_t rbfa_batch_bind(rbfa_batch* b, int32_t nstencils, const int32_t
     }
     std::vector<int> order;
     order.reserve(static_cast<size_t>(nstencils));
+    int start[kMaxClasses] = {}, count[kMaxClasses] = {};
     for (int u = 0; u < b->nclasses; ++u) {
-        b->start[u] = static_cast<int>(order.size());
-        b->count[u] = static_cast<int>(lists[u].size());
+        start[u] = static_cast<int>(order.size());
+        count[u] = static_cast<int>(lists[u].size());
         order.insert(order.end(), lists[u].begin(), lists[u].end());
     }
     drop_graphs(b);
@@ -390,6 +396,10 @@ RBFA_API int32_t rbfa_batch_bind(rbfa_batch* b, int32_t nstencils, const int32_t
                            hipMemcpyHostToDevice, static_cast<hipStream_t>(stream)) != hipSuccess ||
             hipStreamSynchronize(static_cast<hipStream_t>(stream)) != hipSuccess)
             return RBFA_ERR_HIP;
+    }
+    for (int u = 0; u < b->nclasses; ++u) {
+        b->start[u] = start[u];
+        b->count[u] = count[u];
     }
     b->nstencils = nstencils;
     b->ia = ia;

@@ -302,6 +302,100 @@ static int64_t disk_rings(double hc, uint64_t seed, double* x, double* y, int64_
     return cnt;
 }
 
+// Radially graded disk by dart throwing. Target spacing h(r) = hc (1 - a r),
+// a = 3/4 (hc at the centre, hc/4 at the rim); candidates are drawn from the
+// density 1 / h(r)^2 and a candidate p is kept when every node q already
+// kept lies at least beta (h(p) + h(q)) / 2 away. The boundary nodes go in
+// first, so interior nodes keep the same distance from them.
+class GradedDarts {
+  public:
+    static constexpr double a = 0.75;
+    // hard-core distance in units of the local spacing: a packing fraction
+    // pi beta^2 / 4 = 0.40 at one node per h^2, well below the 0.547 at
+    // which random sequential adsorption jams
+    static constexpr double beta = 0.7136;
+
+    GradedDarts(int64_t n_interior, uint64_t seed) : rng_(seed) {
+        // nodes per unit area 1 / h(r)^2: n = 2 pi G(1) / hc^2
+        hc_ = std::sqrt(6.283185307179586 * mass(1.0) / static_cast<double>(n_interior));
+        cell_ = beta * spacing(1.0);
+        side_ = static_cast<int64_t>(std::ceil(2.0 / cell_)) + 1;
+        head_.assign(static_cast<size_t>(side_ * side_), -1);
+    }
+
+    double spacing(double r) const { return hc_ * (1.0 - a * r); }
+
+    // Radius of a candidate: inverts the radial mass G(r) / G(1) by Newton
+    // steps from a bisection bracket.
+    double draw_radius() {
+        const double target = rng_.next() * mass(1.0);
+        double lo = 0.0, hi = 1.0, r = 0.5;
+        for (int it = 0; it < 60; ++it) {
+            const double f = mass(r) - target;
+            if (f > 0) hi = r; else lo = r;
+            const double step = f / (r / ((1.0 - a * r) * (1.0 - a * r)));
+            double nr = r - step;
+            if (!(nr > lo && nr < hi)) nr = 0.5 * (lo + hi);
+            if (std::fabs(nr - r) < 1e-15) { r = nr; break; }
+            r = nr;
+        }
+        return r;
+    }
+    double draw_unit() { return rng_.next(); }
+
+    // Keeps (px, py) unless a kept node is too close; returns whether kept.
+    bool offer(double px, double py) {
+        const double r = std::min(1.0, std::hypot(px, py));
+        const double hp = spacing(r);
+        // the largest spacing a violating neighbour can have: the spacing
+        // nearer the centre, one search radius in
+        const double reach = beta * spacing(std::max(0.0, r - beta * hc_));
+        const int64_t cx = cell_of(px), cy = cell_of(py);
+        const int64_t span = static_cast<int64_t>(std::ceil(reach / cell_));
+        for (int64_t j = std::max<int64_t>(0, cy - span); j <= std::min(side_ - 1, cy + span); ++j)
+            for (int64_t i = std::max<int64_t>(0, cx - span); i <= std::min(side_ - 1, cx + span);
+                 ++i)
+                for (int64_t q = head_[static_cast<size_t>(j * side_ + i)]; q >= 0;
+                     q = next_[static_cast<size_t>(q)]) {
+                    const double dx = px_[static_cast<size_t>(q)] - px;
+                    const double dy = py_[static_cast<size_t>(q)] - py;
+                    const double need = 0.5 * beta * (hp + hq_[static_cast<size_t>(q)]);
+                    if (dx * dx + dy * dy < need * need) return false;
+                }
+        keep(px, py, hp, cx, cy);
+        return true;
+    }
+    void keep_unchecked(double px, double py) {
+        keep(px, py, spacing(std::min(1.0, std::hypot(px, py))), cell_of(px), cell_of(py));
+    }
+    const std::vector<double>& xs() const { return px_; }
+    const std::vector<double>& ys() const { return py_; }
+
+  private:
+    // G(r) = integral_0^r s / (1 - a s)^2 ds
+    static double mass(double r) {
+        const double u = 1.0 - a * r;
+        return (1.0 / u - 1.0 + std::log(u)) / (a * a);
+    }
+    int64_t cell_of(double v) const {
+        return std::min(side_ - 1, std::max<int64_t>(0, static_cast<int64_t>((v + 1.0) / cell_)));
+    }
+    void keep(double px, double py, double hp, int64_t cx, int64_t cy) {
+        const size_t c = static_cast<size_t>(cy * side_ + cx);
+        next_.push_back(head_[c]);
+        head_[c] = static_cast<int64_t>(px_.size());
+        px_.push_back(px);
+        py_.push_back(py);
+        hq_.push_back(hp);
+    }
+
+    XorShift64 rng_;
+    double hc_, cell_;
+    int64_t side_;
+    std::vector<int64_t> head_, next_;
+    std::vector<double> px_, py_, hq_;
+};
+
 }  // namespace
 
 extern "C" {
@@ -439,6 +533,40 @@ RBFA_API int32_t rbfa_gen_jittered_grid(int32_t dim, int32_t m, double jitter, u
     return RBFA_OK;
 }
 
+
+RBFA_API int32_t rbfa_gen_disk_darts(int32_t n_interior, int32_t n_boundary, uint64_t seed,
+                                     double* x, double* y, int32_t* flag) {
+    if (n_interior < 1 || n_boundary < 0 || !x || !y || !flag) return RBFA_ERR_ARG;
+    GradedDarts d(n_interior, seed);
+    const double two_pi = 6.283185307179586;
+    for (int32_t j = 0; j < n_boundary; ++j) {
+        const double th = two_pi * j / n_boundary;
+        d.keep_unchecked(std::cos(th), std::sin(th));
+    }
+    // a dart count far past what the packing fraction needs means the
+    // process has jammed (a boundary far denser than the rim spacing)
+    const int64_t max_darts = 64 * static_cast<int64_t>(n_interior) + 1000;
+    int64_t kept = 0;
+    for (int64_t t = 0; t < max_darts && kept < n_interior; ++t) {
+        const double r = d.draw_radius();
+        const double th = two_pi * d.draw_unit();
+        if (d.offer(r * std::cos(th), r * std::sin(th))) ++kept;
+    }
+    if (kept < n_interior) return RBFA_ERR_ARG;
+    const auto& px = d.xs();
+    const auto& py = d.ys();
+    for (int64_t i = 0; i < n_interior; ++i) {
+        x[i] = px[static_cast<size_t>(n_boundary + i)];
+        y[i] = py[static_cast<size_t>(n_boundary + i)];
+        flag[i] = 0;
+    }
+    for (int32_t j = 0; j < n_boundary; ++j) {
+        x[n_interior + j] = px[static_cast<size_t>(j)];
+        y[n_interior + j] = py[static_cast<size_t>(j)];
+        flag[n_interior + j] = 1;
+    }
+    return RBFA_OK;
+}
 
 RBFA_API int32_t rbfa_gen_graded_disk(int32_t n_interior, int32_t n_boundary, uint64_t seed,
                                       double* x, double* y, int32_t* flag) {

@@ -87,6 +87,7 @@ def lib():
         "rbfa_plan_set_variant": (i32, [P, i32]),
         "rbfa_plan_autotune": (i32, [P, i32, P, P, P, P, P, P, P, P, P, P, P, i32, P]),
         "rbfa_gen_graded_disk": (i32, [i32, i32, u64, P, P, P]),
+        "rbfa_gen_disk_darts": (i32, [i32, i32, u64, P, P, P]),
         "rbfa_batch_create": (i32, [i32, P, P]),
         "rbfa_batch_destroy": (i32, [P]),
         "rbfa_batch_bind": (i32, [P, i32, P, P, P, P, P, P, P, P, P, P]),
@@ -297,6 +298,18 @@ def graded_disk(n_interior, n_boundary, seed=1):
     flag = np.empty(n, np.int32)
     _check(lib().rbfa_gen_graded_disk(int(n_interior), int(n_boundary), int(seed), _ptr(x),
                                       _ptr(y), _ptr(flag)), "rbfa_gen_graded_disk")
+    return x, y, flag
+
+
+def disk_darts(n_interior, n_boundary, seed=1):
+    """Graded unit disk by seeded rejection sampling (spacing hc at the
+    centre to hc/4 at the rim, Poisson-disk hard core) plus n_boundary
+    equispaced rim nodes; returns x, y, flag."""
+    n = int(n_interior) + int(n_boundary)
+    x, y = np.empty(n), np.empty(n)
+    flag = np.empty(n, np.int32)
+    _check(lib().rbfa_gen_disk_darts(int(n_interior), int(n_boundary), int(seed), _ptr(x),
+                                     _ptr(y), _ptr(flag)), "rbfa_gen_disk_darts")
     return x, y, flag
 
 

@@ -9,7 +9,8 @@ import time
 
 import numpy as np
 
-from . import graded_disk, hilbert_order, jittered_grid, knn_stencils, morton_order_3d, ragged_stencils
+from . import (disk_darts, graded_disk, hilbert_order, jittered_grid, knn_stencils, morton_order_3d,
+               ragged_stencils)
 
 SHAPES = {
     # name: (dim, points per axis, n, P, Q, renumbering)
@@ -80,10 +81,18 @@ def disk_sizes(x, y, flag):
     return n
 
 
-def build_disk(n_interior=2_000_000, n_boundary=8000, seed=5):
-    """Graded unit disk, Hilbert-renumbered, ragged kNN stencils by class."""
+def build_disk(n_interior=2_000_000, n_boundary=8000, seed=5, method="darts"):
+    """Graded unit disk, Hilbert-renumbered, ragged kNN stencils by class.
+    method "darts": seeded rejection sampling with the 1:4 radial spacing
+    (rbfa_gen_disk_darts); "rings": jittered concentric rings of the same
+    grading (rbfa_gen_graded_disk)."""
     t0 = time.perf_counter()
-    x, y, flag = graded_disk(n_interior, n_boundary, seed)
+    if method == "darts":
+        x, y, flag = disk_darts(n_interior, n_boundary, seed)
+    elif method == "rings":
+        x, y, flag = graded_disk(n_interior, n_boundary, seed)
+    else:
+        raise ValueError(f"unknown disk method {method!r}")
     t1 = time.perf_counter()
     p = hilbert_order(x, y, 16)
     x, y, flag = p.permute(x), p.permute(y), p.permute(flag)

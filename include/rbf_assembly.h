@@ -154,6 +154,18 @@ RBFA_API int32_t rbfa_gen_jittered_grid(int32_t dim, int32_t m, double jitter, u
 RBFA_API int32_t rbfa_gen_graded_disk(int32_t n_interior, int32_t n_boundary, uint64_t seed,
                                       double* x, double* y, int32_t* flag);
 
+/* Graded unit disk by seeded rejection sampling (dart throwing): target
+ * spacing h(r) = hc (1 - 0.75 r), spacing ratio 1:4 from centre to rim, hc
+ * solved for n_interior nodes at one node per h^2. Candidates are drawn
+ * from that density and kept when every kept node q lies at least
+ * 0.7136 (h(p) + h(q)) / 2 away (a Poisson-disk set at packing fraction
+ * 0.40); the n_boundary equispaced nodes on the unit circle are placed
+ * first, so the interior keeps its distance from them too. Same output
+ * layout as rbfa_gen_graded_disk. RBFA_ERR_ARG if the darts jam before
+ * n_interior nodes (a boundary far denser than the rim spacing). */
+RBFA_API int32_t rbfa_gen_disk_darts(int32_t n_interior, int32_t n_boundary, uint64_t seed,
+                                     double* x, double* y, int32_t* flag);
+
 /* ---- the node layer on the device (device pointers, stream-ordered) -----
  * Same results as the host functions above, bit for bit, for the same
  * points: the same keys, the same stable order, the same stencils (the

@@ -288,6 +288,43 @@ def test_uneven_batch_autotune():
     assert all(b.class_variant(c) for c in range(4))
 
 
+def test_uneven_batch_failed_bind_leaves_it_unbound():
+    """A bind that fails (a row length no class has) leaves the batch
+    unbound: launch and autotune refuse it until a bind succeeds."""
+    import torch
+    from rbf_amd import workloads
+    W = _disk_case(3000, 120, 23)
+    nnz = int(W["ia"][-1])
+    b = R.Batch(workloads.DISK_CLASSES)
+    w = [torch.zeros(nnz, dtype=torch.float64, device="cuda:0") for _ in range(3)]
+    args = (_dev(W["x"]), _dev(W["y"]), _dev(W["ia"]), _dev(W["ja"]), w)
+    b.bind(W["ia"], *args)
+    b.launch()
+    torch.cuda.synchronize()
+    bad = W["ia"].copy()
+    bad[1:] += 1  # row 0 one entry longer: no class of that size
+    with pytest.raises(R.RbfError) as e:
+        b.bind(bad, *args)
+    assert e.value.status == R.ERR_ARG
+    with pytest.raises(R.RbfError):
+        b.launch()
+    with pytest.raises(R.RbfError):
+        b.autotune(reps=1)
+    b.bind(W["ia"], *args)
+    for t in w:
+        t.zero_()
+    b.launch()
+    torch.cuda.synchronize()
+    wref = [np.zeros(nnz) for _ in range(3)]
+    iref = np.zeros(W["x"].size, np.int32)
+    for (n, P, Q) in workloads.DISK_CLASSES:
+        rows = np.nonzero(W["sizes"] == n)[0].astype(np.int32)
+        O.assemble_rows(2, n, P, Q, rows, W["ia"], W["x"], W["y"], W["ja"], wref, iref)
+    for a, r in zip(w, wref):
+        g = a.cpu().numpy()
+        assert np.abs(g - r).max() <= FP64_TOL * np.abs(r).max()
+
+
 @pytest.mark.parametrize("dim,m,n,P,Q", [(2, 32, 56, 4, 7), (3, 10, 80, 3, 3)])
 def test_large_systems_mixed(dim, m, n, P, Q):
     """Mixed precision for systems larger than 64: within 1e-8 of the FP64

@@ -242,6 +242,60 @@ def test_graded_disk_and_ragged_stencils():
         assert jr[ia[s]] == s
 
 
+def test_disk_darts_rejection_sampling():
+    """Seeded rejection sampling on the unit disk: spacing h(r) = hc (1 -
+    0.75 r) (1:4 centre to rim), one node per h^2, a hard core of
+    0.7136 (h(p) + h(q)) / 2 between every pair, equispaced rim nodes."""
+    from scipy.spatial import cKDTree
+    from rbf_amd import workloads
+    n, nb = 40000, 800
+    x, y, flag = R.disk_darts(n, nb, seed=9)
+    assert x.size == n + nb and (flag[:n] == 0).all() and (flag[n:] == 1).all()
+    r = np.hypot(x, y)
+    assert r[:n].max() < 1.0
+    th = np.arctan2(y[n:], x[n:]) % (2 * np.pi)
+    np.testing.assert_allclose(r[n:], 1.0, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(np.sort(th), 2 * np.pi * np.arange(nb) / nb, atol=1e-12)
+    x2, y2, _ = R.disk_darts(n, nb, seed=9)
+    np.testing.assert_array_equal(x, x2)
+    np.testing.assert_array_equal(y, y2)
+    assert not np.array_equal(R.disk_darts(n, nb, seed=10)[0], x)
+
+    a = 0.75
+
+    def mass(rr):  # integral_0^r s / (1 - a s)^2 ds
+        u = 1 - a * rr
+        return (1 / u - 1 + np.log(u)) / a ** 2
+
+    hc = np.sqrt(2 * np.pi * mass(1.0) / n)
+    h = hc * (1 - a * np.minimum(r, 1.0))
+    # the hard core, over every pair that could violate it
+    pairs = np.array(sorted(cKDTree(np.c_[x, y]).query_pairs(0.7136 * hc)))
+    d = np.hypot(x[pairs[:, 0]] - x[pairs[:, 1]], y[pairs[:, 0]] - y[pairs[:, 1]])
+    assert (d >= 0.7136 * 0.5 * (h[pairs[:, 0]] + h[pairs[:, 1]]) * (1 - 1e-12)).all()
+    # the radial distribution is the target density's: fraction inside rho
+    # is G(rho) / G(1)
+    ri = r[:n]
+    for rho in (0.25, 0.5, 0.75, 0.9):
+        assert abs((ri < rho).mean() - mass(rho) / mass(1.0)) < 0.01, rho
+    # nearest-neighbour spacing grades with h: centre / rim ~ 0.9625 / 0.2875
+    ja = R.knn_stencils(2, x, y)[1::2]
+    nn = np.hypot(x - x[ja], y - y[ja])
+    ratio = np.median(nn[r < 0.1]) / np.median(nn[(r > 0.9) & (flag == 0)])
+    assert 0.9 * 3.35 < ratio < 1.1 * 3.35
+    sizes = workloads.disk_sizes(x, y, flag)
+    assert set(np.unique(sizes)) == {13, 21, 35, 56}
+
+
+def test_disk_darts_rejects_bad_arguments():
+    with pytest.raises(R.RbfError):
+        R.disk_darts(0, 10)
+    # a boundary far denser than the rim spacing jams the darts: an error,
+    # not a short node set
+    with pytest.raises(R.RbfError):
+        R.disk_darts(100, 20000)
+
+
 def test_oracle_ragged_rows_equal_fixed_width():
     x, y = R.jittered_grid(2, 24, 0.25, seed=4)
     ja = R.knn_stencils(21, x, y)
