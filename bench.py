@@ -19,9 +19,12 @@ Other workloads (--workload), reported in the same format:
   square_4k_n21    4,096 jittered nodes, n = 21, r^3 + P2 (27x27)
   square_1m_n56    1M jittered nodes, n = 56, r^7 + P4 (71x71)
   cube_2m_n80      128^3 jittered unit cube, Morton, n = 80, r^3 + P3 (100x100), 4 operators
-  disk_2m_uneven   graded unit disk, 2,000,000 interior + 8,000 boundary nodes,
-                   n in {13, 21, 35, 56} by size class, one ragged graph,
-                   captured hipGraph (per-class launches or one persistent launch)
+  disk_2m_uneven   graded unit disk, 2,000,000 interior nodes drawn by seeded
+                   rejection sampling (spacing 1:4 centre to rim, Poisson-disk
+                   hard core; --disk rings: jittered rings instead) + 8,000
+                   equispaced boundary nodes, n in {13, 21, 35, 56} by size
+                   class, one ragged graph, captured hipGraph (per-class
+                   launches or one persistent launch)
   nodes_1m_n35     the node layer on the device for the headline cloud: Hilbert
                    keys, stable order, coordinate gathers and n = 35 kNN
                    stencils (bit-identical to the reference's hilbert_order +
@@ -406,7 +409,7 @@ def oracle_fixed(W, res, n, dim, sample):
 
 
 def run_disk(args, R, workloads, torch, dev):
-    W = workloads.build_disk(seed=args.seed + 3)
+    W = workloads.build_disk(seed=args.seed + 3, method=args.disk)
     log(f"[disk] host prep: {W['host_s']}")
     nst, ia, jah = W["nstencils"], W["ia"], W["ja"]
     nnz = int(ia[-1])
@@ -470,6 +473,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline and parity")
     ap.add_argument("--form", choices=["both", "fp64", "mixed"], default="both")
     ap.add_argument("--no-tune", action="store_true", help="skip the variant autotuner")
+    ap.add_argument("--disk", choices=["darts", "rings"], default="darts",
+                    help="node generator of disk_2m_uneven")
     ap.add_argument("--variant", type=int, default=None,
                     help="use this kernel variant (no autotuning)")
     ap.add_argument("--profile-copy", action="store_true",
@@ -637,6 +642,9 @@ def main():
         "forms": extra,
         "host_prep_s": W["host_s"],
     }
+    if name == "disk_2m_uneven":
+        line["config"]["generator"] = ("rejection sampling, 1:4 radial spacing" if args.disk ==
+                                       "darts" else "jittered rings, 1:4 radial spacing")
     if name == HEADLINE:
         line["config"]["workload"] = ("square_1m_n35: 1024x1024 jittered unit square, Hilbert, "
                                       "n=35, PHS r^5 + P3 (45x45), Laplacian/dx/dy, form=" + head)
