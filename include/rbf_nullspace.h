@@ -91,9 +91,12 @@ struct NullSpaceMixed {
     // pivot rows of P's LU, A_pipi, t_pi and H; the padding rows of G
     // (ranks f .. TP-1, zero, read only by the MFMA products) take the
     // place of P's pivot rows once the particular solution has used them;
-    // after the Schur product region 1 holds S (FP64, row-major).
+    // after the Schur product region 1 holds S (FP64): row-major, or, for
+    // the systems that list their fallbacks (where the LDS of a wave sets
+    // the waves per CU), its upper triangle packed by rows.
     static constexpr int GS = M + 1;  // row stride of G and H (spreads the banks)
     static constexpr int SS = F;      // row stride of S
+    static constexpr bool kPackS = !kInWaveFallback;
     static constexpr int CB = 64;  // floats per published column (every lane stores)
     static constexpr int o_qf = 0;                   // [DIM][TP] free points by rank (0-padded)
     static constexpr int o_y = o_qf + DIM * TP;      // [F][R] reduced right-hand sides
@@ -113,10 +116,21 @@ struct NullSpaceMixed {
     static constexpr int o_t = o_app + M * M;        // [M][R] t at the pivot nodes
     static constexpr int o_h = o_t + M * R;          // [TP][GS] H by rank (0-padded)
     static constexpr int r1_a = o_h + TP * GS - o_r1;
-    static constexpr int r1_b = F * SS;
+    static constexpr int r1_b = kPackS ? F * (F + 1) / 2 : F * SS;
     static constexpr int o_s = o_r1;
     static constexpr int ns_doubles = o_r1 + (r1_a > r1_b ? r1_a : r1_b);
     static constexpr int lds_doubles = ((ns_doubles > fb_lds ? ns_doubles : fb_lds) + 1) / 2 * 2;
+
+    // Offset of S(i, j) in region 1 (packed: row i of the upper triangle
+    // starts at i F - i (i - 1) / 2; S(i, j) = S(j, i))
+    __device__ __forceinline__ static int s_at(int i, int j) {
+        if constexpr (kPackS) {
+            const int r = i < j ? i : j, c = i < j ? j : i;
+            return r * F - (r * (r - 1)) / 2 + (c - r);
+        } else {
+            return i * SS + j;
+        }
+    }
 
     // r^E from r^2 for odd E: the v_rsq_f64 seed refined to the square
     // root as the compiler's own expansion does, without its rescaling of
@@ -539,7 +553,9 @@ struct NullSpaceMixed {
 #pragma unroll
                     for (int v = 0; v < 4; ++v) {
                         const int row = 16 * I + lg + 4 * v;
-                        if (row < F && col < F) {
+                        if constexpr (kPackS) {
+                            if (row <= col && col < F) sb[s_at(row, col)] = acc[ti][v];
+                        } else if (row < F && col < F) {
                             sb[row * SS + col] = acc[ti][v];
                             if (I != J) sb[col * SS + row] = acc[ti][v];
                         }
@@ -556,7 +572,7 @@ struct NullSpaceMixed {
         float rowmax = 0.f;
 #pragma unroll
         for (int j = 0; j < F; ++j) {
-            a32[j] = is_row ? static_cast<float>(sb[lane * SS + j]) : 0.f;
+            a32[j] = is_row ? static_cast<float>(sb[s_at(lane, j)]) : 0.f;
             rowmax = fmaxf(rowmax, fabsf(a32[j]));
         }
 #pragma unroll
@@ -604,7 +620,7 @@ struct NullSpaceMixed {
                 if (is_row) {
 #pragma unroll
                     for (int j = 0; j < F; ++j) {
-                        const double sij = sb[lane * SS + j];
+                        const double sij = sb[s_at(lane, j)];
 #pragma unroll
                         for (int c = 0; c < R; ++c) rr[c] = fma(-sij, vb[j * R + c], rr[c]);
                     }
