@@ -155,11 +155,38 @@ struct InterpEng {
     using E = detail::WaveAssembly<2, N, P, Q, R, true, T>;
 };
 
+// Shapes of at most 32 unknowns also run packed, 64 / SL stencils per wave
+// (include/rbf_wavepack.h), listed after the single-wave variants.
+template <class Base, int DIM, int N, int P, int Q, int R, bool INTERP>
+struct WithPack {
+    static constexpr int NT = N + npoly_dim(DIM, P);
+    static constexpr int SL = NT <= 16 ? 16 : 32;
+    static_assert(NT <= 32, "a system per segment of at most 32 lanes");
+    // registers sized for the waves per SIMD they allow, or for one more
+    static constexpr int OCC2 = SL == 16 ? 4 : 3;
+    static constexpr Variant list[kVariants + 2] = {
+        Base::list[0], Base::list[1], Base::list[2], Base::list[3],
+        Base::list[4], Base::list[5], Base::list[6], Base::list[7],
+        {SL == 16 ? "packed: 4 stencils per wave (16-lane segments)"
+                  : "packed: 2 stencils per wave (32-lane segments)",
+         &launch_pack<DIM, N, P, Q, R, INTERP, SL, 0, 1>},
+        {SL == 16 ? "packed: 4 stencils per wave (16-lane segments) occ=4"
+                  : "packed: 2 stencils per wave (32-lane segments) occ=3",
+         &launch_pack<DIM, N, P, Q, R, INTERP, SL, 0, OCC2>}};
+};
+
 template <int DIM, int N, int P, int Q>
 constexpr Shape diff64() {
     using E = detail::WaveAssembly<DIM, N, P, Q, DIM + 1, false>;
     return {DIM, N, P, Q, RBFA_OPS_DIFF, 0, RBFA_PREC_FP64, E::NT, DIM + 1,
             VariantSet<Fp64L, DiffEng<DIM, N, P, Q>::template E>::list, kVariants};
+}
+template <int DIM, int N, int P, int Q>
+constexpr Shape diff64p() {
+    using E = detail::WaveAssembly<DIM, N, P, Q, DIM + 1, false>;
+    using Base = VariantSet<Fp64L, DiffEng<DIM, N, P, Q>::template E>;
+    return {DIM, N, P, Q, RBFA_OPS_DIFF, 0, RBFA_PREC_FP64, E::NT, DIM + 1,
+            WithPack<Base, DIM, N, P, Q, DIM + 1, false>::list, kVariants + 2};
 }
 template <int DIM, int N, int P, int Q>
 struct GroupVariants {
@@ -202,8 +229,9 @@ constexpr Shape diffgroup() {
 template <int N, int P, int Q, int R>
 constexpr Shape interp64() {
     using E = detail::WaveAssembly<2, N, P, Q, R, true>;
+    using Base = VariantSet<Fp64L, InterpEng<N, P, Q, R>::template E>;
     return {2, N, P, Q, RBFA_OPS_INTERP, R, RBFA_PREC_FP64, E::NT, R,
-            VariantSet<Fp64L, InterpEng<N, P, Q, R>::template E>::list, kVariants};
+            WithPack<Base, 2, N, P, Q, R, true>::list, kVariants + 2};
 }
 // Mixed precision: the null-space solver (FP64 polynomial-block LU and
 // Schur complement on FP64 MFMA, FP32 LU of the reduced system, FP64
@@ -239,10 +267,11 @@ constexpr Shape diffmix() {
 // PHS, degree 2-3 polynomials) and the D2Q9 interpolation stencils of the
 // reference's CUDA-Fortran demo (n = 9, NRHS = 9, main.f90:16).
 const Shape kShapes[] = {
-    diff64<2, 13, 2, 3>(),  diff64<2, 21, 2, 3>(), diff64<2, 35, 3, 5>(),
-    diff64<2, 35, 3, 3>(),  diff64<2, 21, 2, 5>(), diff64<2, 13, 1, 3>(),
+    diff64p<2, 13, 2, 3>(), diff64p<2, 21, 2, 3>(), diff64<2, 35, 3, 5>(),
+    diff64<2, 35, 3, 3>(),  diff64p<2, 21, 2, 5>(), diff64p<2, 13, 1, 3>(),
     interp64<9, 2, 3, 9>(), interp64<21, 2, 3, 9>(),
     diffmix<2, 21, 2, 3>(), diffmix<2, 35, 3, 5>(), diffmix<2, 13, 2, 3>(),
+    diffmix<2, 35, 3, 3>(), diffmix<2, 21, 2, 5>(), diffmix<2, 13, 1, 3>(),
     // systems larger than a wavefront: one two-wave workgroup per stencil
     diffgroup<2, 56, 4, 7>(), diffgroup<3, 80, 3, 3>(),
     diffnsmix<2, 56, 4, 7>(), diffgroupmix<3, 80, 3, 3>(),
