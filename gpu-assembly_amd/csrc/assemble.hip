@@ -162,9 +162,10 @@ struct WithPack {
     static constexpr int NT = N + npoly_dim(DIM, P);
     static constexpr int SL = NT <= 16 ? 16 : 32;
     static_assert(NT <= 32, "a system per segment of at most 32 lanes");
-    // registers sized for the waves per SIMD they allow, or for one more
+    // registers sized for the waves per SIMD they allow, or for one more;
+    // the pivot row read by LDS broadcast, or per lane + DPP row_newbcast
     static constexpr int OCC2 = SL == 16 ? 4 : 3;
-    static constexpr Variant list[kVariants + 2] = {
+    static constexpr Variant list[kVariants + 4] = {
         Base::list[0], Base::list[1], Base::list[2], Base::list[3],
         Base::list[4], Base::list[5], Base::list[6], Base::list[7],
         {SL == 16 ? "packed: 4 stencils per wave (16-lane segments)"
@@ -172,7 +173,13 @@ struct WithPack {
          &launch_pack<DIM, N, P, Q, R, INTERP, SL, 0, 1>},
         {SL == 16 ? "packed: 4 stencils per wave (16-lane segments) occ=4"
                   : "packed: 2 stencils per wave (32-lane segments) occ=3",
-         &launch_pack<DIM, N, P, Q, R, INTERP, SL, 0, OCC2>}};
+         &launch_pack<DIM, N, P, Q, R, INTERP, SL, 0, OCC2>},
+        {SL == 16 ? "packed: 4 stencils per wave (16-lane segments) bc=newbcast"
+                  : "packed: 2 stencils per wave (32-lane segments) bc=newbcast",
+         &launch_pack<DIM, N, P, Q, R, INTERP, SL, 1, 1>},
+        {SL == 16 ? "packed: 4 stencils per wave (16-lane segments) bc=newbcast occ=4"
+                  : "packed: 2 stencils per wave (32-lane segments) bc=newbcast occ=3",
+         &launch_pack<DIM, N, P, Q, R, INTERP, SL, 1, OCC2>}};
 };
 
 template <int DIM, int N, int P, int Q>
@@ -186,7 +193,7 @@ constexpr Shape diff64p() {
     using E = detail::WaveAssembly<DIM, N, P, Q, DIM + 1, false>;
     using Base = VariantSet<Fp64L, DiffEng<DIM, N, P, Q>::template E>;
     return {DIM, N, P, Q, RBFA_OPS_DIFF, 0, RBFA_PREC_FP64, E::NT, DIM + 1,
-            WithPack<Base, DIM, N, P, Q, DIM + 1, false>::list, kVariants + 2};
+            WithPack<Base, DIM, N, P, Q, DIM + 1, false>::list, kVariants + 4};
 }
 template <int DIM, int N, int P, int Q>
 struct GroupVariants {
@@ -231,7 +238,7 @@ constexpr Shape interp64() {
     using E = detail::WaveAssembly<2, N, P, Q, R, true>;
     using Base = VariantSet<Fp64L, InterpEng<N, P, Q, R>::template E>;
     return {2, N, P, Q, RBFA_OPS_INTERP, R, RBFA_PREC_FP64, E::NT, R,
-            WithPack<Base, 2, N, P, Q, R, true>::list, kVariants + 2};
+            WithPack<Base, 2, N, P, Q, R, true>::list, kVariants + 4};
 }
 // Mixed precision: the null-space solver (FP64 polynomial-block LU and
 // Schur complement on FP64 MFMA, FP32 LU of the reduced system, FP64

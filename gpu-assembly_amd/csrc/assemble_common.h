@@ -55,10 +55,10 @@ hipError_t launch_ns_list(const LaunchArgs& a);
 template <int DIM, int N, int P, int Q>
 hipError_t launch_tile_diff(const LaunchArgs& a);
 // Systems of at most 32 unknowns, SL-lane segments, one stencil per
-// segment, 64 / SL stencils per wave (assemble_pack.hip); CH: columns per
-// scheduling chunk of the row update (0: one chunk), OCC: waves per SIMD
-// the registers are sized for.
-template <int DIM, int N, int P, int Q, int R, bool INTERP, int SL, int CH, int OCC>
+// segment, 64 / SL stencils per wave (assemble_pack.hip); BCM: pivot-row
+// broadcast (0 LDS broadcast reads, 1 per-lane reads + DPP row_newbcast),
+// OCC: waves per SIMD the registers are sized for.
+template <int DIM, int N, int P, int Q, int R, bool INTERP, int SL, int BCM, int OCC>
 hipError_t launch_pack(const LaunchArgs& a);
 
 }  // namespace rbf_amd_internal

@@ -33,10 +33,10 @@ k_assemble_pack(int nstencils, const double* __restrict__ x, const double* __res
 
 }  // namespace
 
-template <int DIM, int N, int P, int Q, int R, bool INTERP, int SL, int CH, int OCC>
+template <int DIM, int N, int P, int Q, int R, bool INTERP, int SL, int BCM, int OCC>
 hipError_t launch_pack(const LaunchArgs& a) {
     constexpr int WPB = 1;
-    using E = detail::WavePack<DIM, N, P, Q, R, INTERP, SL, CH>;
+    using E = detail::WavePack<DIM, N, P, Q, R, INTERP, SL, 0, BCM>;
     detail::ValueOut<double, R> out;
     for (int c = 0; c < R; ++c) out.p[c] = a.w[c];
     const int nwaves = (a.nstencils + E::S - 1) / E::S;
@@ -49,11 +49,13 @@ hipError_t launch_pack(const LaunchArgs& a) {
 
 // derivative sets (Laplacian, d/dx, d/dy) of the 2-D shapes up to 32
 // unknowns, and interpolation to the D2Q9 lattice points (n = 9: 15 x 15;
-// n = 21: 27 x 27); registers unconstrained, and sized for 4 (16-lane
-// segments) or 3 (32-lane segments) waves per SIMD
-#define RBFA_PACK(N, P, Q, R, INTERP, SL, OCC2)                                   \
-    template hipError_t launch_pack<2, N, P, Q, R, INTERP, SL, 0, 1>(const LaunchArgs&); \
-    template hipError_t launch_pack<2, N, P, Q, R, INTERP, SL, 0, OCC2>(const LaunchArgs&);
+// n = 21: 27 x 27); both broadcasts, registers unconstrained and sized for
+// 4 (16-lane segments) or 3 (32-lane segments) waves per SIMD
+#define RBFA_PACK(N, P, Q, R, INTERP, SL, OCC2)                                       \
+    template hipError_t launch_pack<2, N, P, Q, R, INTERP, SL, 0, 1>(const LaunchArgs&);    \
+    template hipError_t launch_pack<2, N, P, Q, R, INTERP, SL, 0, OCC2>(const LaunchArgs&); \
+    template hipError_t launch_pack<2, N, P, Q, R, INTERP, SL, 1, 1>(const LaunchArgs&);    \
+    template hipError_t launch_pack<2, N, P, Q, R, INTERP, SL, 1, OCC2>(const LaunchArgs&);
 RBFA_PACK(13, 1, 3, 3, false, 16, 4)
 RBFA_PACK(13, 2, 3, 3, false, 32, 3)
 RBFA_PACK(21, 2, 3, 3, false, 32, 3)

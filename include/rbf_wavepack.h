@@ -55,7 +55,7 @@ __device__ __forceinline__ double seg_max_nonneg(double v) {
     return __hiloint2double(static_cast<int>(mh), static_cast<int>(ml));
 }
 
-template <int DIM, int N, int P, int Q, int R, bool INTERP, int SL, int CH = 0>
+template <int DIM, int N, int P, int Q, int R, bool INTERP, int SL, int CH = 0, int BCM = 0>
 struct WavePack {
     static constexpr int M = npoly_dim(DIM, P);
     static constexpr int NT = N + M;
@@ -87,7 +87,17 @@ struct WavePack {
     static constexpr int o_cen = o_poly + M * N;
     static constexpr int o_cmono = o_cen + (INTERP ? DIM * R : 0);
     static constexpr int fill_doubles = o_cmono + (INTERP ? R * M : 0);
-    static constexpr int PS = 2 * NC;  // a pivot-row slot (see slot_at)
+    // BCM: how the segment's lanes read the published pivot row.
+    //   0: every lane reads every column at the segment's address (LDS
+    //      broadcast: 64 lanes x 8 B per column and instruction);
+    //   1: lane j of each 16-lane row reads column 16c + j of chunk c (one
+    //      read per chunk of 16 columns), and column j reaches the row's
+    //      lanes by DPP row_newbcast:j from lane j (v_mov_b64_dpp, one VALU
+    //      instruction) -- far fewer LDS cycles for one VALU per column.
+    static_assert(BCM == 0 || BCM == 1, "broadcast mode");
+    static_assert(BCM == 0 || R <= 16, "t of the back substitution in one chunk");
+    static constexpr int NCH = (NC + 15) / 16;  // chunks of 16 columns
+    static constexpr int PS = BCM == 1 ? 32 * NCH : 2 * NC;  // a pivot-row slot (see slot_at)
     static_assert((R + 1) / 2 * 2 <= PS, "back-substitution slots fit the pivot slots");
     static constexpr int seg_doubles = (cmax(fill_doubles, 2 * PS) + 1) / 2 * 2;
     static constexpr int lds_doubles = S * seg_doubles;
@@ -289,6 +299,42 @@ struct WavePack {
         }
     }
 
+    // value of v in lane n of this lane's 16-lane row (DPP row_newbcast)
+    template <int n>
+    __device__ __forceinline__ static double row_bcast(double v) {
+        return __builtin_amdgcn_update_dpp(0.0, v, 0x150 + n, 0xF, 0xF, false);
+    }
+
+    // BCM = 1: this lane's columns of every chunk that still holds a column
+    // >= K0 (lane j of a row reads column 16c + j)
+    template <int K0, int C>
+    __device__ __forceinline__ static void read_chunks(double (&uc)[NCH], const double* pb, int lj) {
+        if constexpr (C < NCH) {
+            if constexpr (16 * C + 15 >= K0) uc[C] = pb[slot_at(16 * C + lj)];
+            read_chunks<K0, C + 1>(uc, pb, lj);
+        }
+    }
+
+    // row[j] = fma(-l, u_j, row[j]) for j = J .. NC-1, u_j by row_newbcast
+    template <int J>
+    __device__ __forceinline__ static void update_dpp(double (&row)[NC], double l,
+                                                      const double (&uc)[NCH]) {
+        if constexpr (J < NC) {
+            row[J] = fma(-l, row_bcast<J % 16>(uc[J / 16]), row[J]);
+            if constexpr (CH > 0 && (J + 1) % CH == 0) __builtin_amdgcn_sched_barrier(0);
+            update_dpp<J + 1>(row, l, uc);
+        }
+    }
+
+    // BCM = 1 back substitution: b_c = fma(-u, t_c, b_c), t_c from lane c
+    template <int C>
+    __device__ __forceinline__ static void back_dpp(double (&row)[NC], double u, double tv) {
+        if constexpr (C < R) {
+            row[NT + C] = fma(-u, row_bcast<C>(tv), row[NT + C]);
+            back_dpp<C + 1>(row, u, tv);
+        }
+    }
+
     template <int K>
     __device__ __forceinline__ static void elim(double (&row)[NC], unsigned long long& done,
                                                 int& mystep, double& myrcp, int& info,
@@ -320,7 +366,14 @@ struct WavePack {
             double* pb = slots + (K & 1) * PS;
             if (is_piv) publish<K, K>(pb, row);
             wave_lds_sync();
-            const double piv = pb[slot_at(K)];
+            double uc[NCH];
+            double piv;
+            if constexpr (BCM == 1) {
+                read_chunks<K, 0>(uc, pb, lane_id() & 15);
+                piv = row_bcast<K % 16>(uc[K / 16]);
+            } else {
+                piv = pb[slot_at(K)];
+            }
             if (piv == 0.0 && !info) info = K + 1;
             const double rcp = 1.0 / piv;
             const bool elim_row = lane_in(act & ~pm);
@@ -329,7 +382,10 @@ struct WavePack {
             mystep = is_piv ? K : mystep;
             myrcp = is_piv ? rcp : myrcp;
             done |= pm;
-            update<K, K + 1>(row, l, pb);
+            if constexpr (BCM == 1)
+                update_dpp<K + 1>(row, l, uc);
+            else
+                update<K, K + 1>(row, l, pb);
             elim<K + 1>(row, done, mystep, myrcp, info, slots);
         }
     }
@@ -354,14 +410,18 @@ struct WavePack {
                 for (int c = 0; c < R; ++c) tb[c] = row[NT + c] * myrcp;
             }
             wave_lds_sync();
+            if constexpr (BCM == 1) {
+                back_dpp<0>(row, row[k], tb[lane_id() & 15]);
+            } else {
 #pragma unroll
-            for (int c = 0; c < R; c += 2) {
-                if (c + 1 < R) {
-                    const double2 t = *reinterpret_cast<const double2*>(tb + c);
-                    row[NT + c] = fma(-row[k], t.x, row[NT + c]);
-                    row[NT + c + 1] = fma(-row[k], t.y, row[NT + c + 1]);
-                } else {
-                    row[NT + c] = fma(-row[k], tb[c], row[NT + c]);
+                for (int c = 0; c < R; c += 2) {
+                    if (c + 1 < R) {
+                        const double2 t = *reinterpret_cast<const double2*>(tb + c);
+                        row[NT + c] = fma(-row[k], t.x, row[NT + c]);
+                        row[NT + c + 1] = fma(-row[k], t.y, row[NT + c + 1]);
+                    } else {
+                        row[NT + c] = fma(-row[k], tb[c], row[NT + c]);
+                    }
                 }
             }
             static_assert(TS <= PS, "slots");
