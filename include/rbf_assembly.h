@@ -234,7 +234,13 @@ RBFA_API int32_t rbfa_plan_system_size(const rbfa_plan* plan);
  *
  * Replaces the launch of assemble_interp_weights<Config>
  * (rbf_operators.h:317-328) and the sketched assemble_laplace_weights
- * (main.cpp:44-45). Asynchronous: returns once the work is enqueued. */
+ * (main.cpp:44-45). Asynchronous: returns once the work is enqueued.
+ * Mixed-form plans of systems above 64 unknowns (2-D n = 56, 3-D n = 80)
+ * keep their FP64-fallback list in a workspace of the plan (grown here,
+ * with a synchronisation of `stream` when it grows): calls with one such
+ * plan must not overlap (one stream, one host thread at a time), or a
+ * call can clear the other's list and leave its listed stencils unsolved.
+ * Every other plan keeps no state between calls. */
 RBFA_API int32_t rbfa_assemble(const rbfa_plan* plan, int32_t nstencils, const double* x,
                                const double* y, const double* z, const int32_t* ja,
                                double* const* w, const double* xc, const double* yc,
