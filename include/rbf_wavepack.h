@@ -235,7 +235,7 @@ struct WavePack {
 
     // some segment of x holds no set bit (x - LOWB borrows into a segment's
     // top bit exactly where the segment is zero: the classic zero-lane test)
-    __device__ __forceinline__ static bool has_empty_segment(unsigned long long x) {
+    __host__ __device__ __forceinline__ static bool has_empty_segment(unsigned long long x) {
         return ((x - LOWB) & ~x & HIGHB) != 0;
     }
 
@@ -244,7 +244,7 @@ struct WavePack {
     // (every active entry NaN), its first active lane, that segment then
     // marked in nocand. (With every segment nonempty, hit & (~hit + LOWB)
     // gives the same mask: the carries stop inside each segment.)
-    __device__ __forceinline__ static unsigned long long first_per_segment(
+    __host__ __device__ __forceinline__ static unsigned long long first_per_segment(
         unsigned long long hit, unsigned long long act, unsigned long long& nocand) {
         unsigned long long pm = 0;
         nocand = 0;
@@ -259,7 +259,7 @@ struct WavePack {
         return pm;
     }
 
-    __device__ __forceinline__ static bool some_segment_tied(unsigned long long hit) {
+    __host__ __device__ __forceinline__ static bool some_segment_tied(unsigned long long hit) {
         bool tied = false;
 #pragma unroll
         for (int g = 0; g < S; ++g) {
@@ -267,6 +267,21 @@ struct WavePack {
             tied = tied || (hg & (hg - 1)) != 0;
         }
         return tied;
+    }
+
+    // The two mask steps of a pivot search, from the ballot `hit` of the
+    // lanes holding their segment's top key: whether some segment has more
+    // than one such lane (then the low words decide), and the pivot lane of
+    // every segment (nocand: the segments with no candidate). The usual
+    // case, a candidate in every segment, takes a few scalar operations on
+    // the whole mask; the segment-by-segment loops only run otherwise.
+    __host__ __device__ __forceinline__ static bool segments_tied(unsigned long long hit) {
+        return !has_empty_segment(hit) ? (hit & (hit - LOWB)) != 0 : some_segment_tied(hit);
+    }
+    __host__ __device__ __forceinline__ static unsigned long long pivot_lanes(
+        unsigned long long hit, unsigned long long act, unsigned long long& nocand) {
+        nocand = 0;
+        return !has_empty_segment(hit) ? hit & (~hit + LOWB) : first_per_segment(hit, act, nocand);
     }
 
     // Slot position of pivot-row column j: every other double, so no two
@@ -349,18 +364,13 @@ struct WavePack {
             const unsigned hi = lane_in(cm) ? static_cast<unsigned>(__double2hiint(fa)) : 0u;
             const unsigned mh = seg_umax<SL>(hi);
             unsigned long long hit = cm & wave_ballot(hi == mh);
-            // the usual case, a candidate in every segment, in a few scalar
-            // operations on the whole mask; segment by segment otherwise
-            const bool full = !has_empty_segment(hit);
-            const bool tied = full ? (hit & (hit - LOWB)) != 0 : some_segment_tied(hit);
-            if (tied) {
+            if (segments_tied(hit)) {
                 const unsigned lo = lane_in(hit) ? static_cast<unsigned>(__double2loint(fa)) : 0u;
                 const unsigned ml = seg_umax<SL>(lo);
                 hit &= wave_ballot(lo == ml);
             }
-            unsigned long long nocand = 0;
-            const unsigned long long pm =
-                full ? hit & (~hit + LOWB) : first_per_segment(hit, act, nocand);
+            unsigned long long nocand;
+            const unsigned long long pm = pivot_lanes(hit, act, nocand);
             if (lane_in(nocand) && !info) info = K + 1;
             const bool is_piv = lane_in(pm);
             double* pb = slots + (K & 1) * PS;
