@@ -126,7 +126,9 @@ struct WavePack {
         const double pz = DIM == 3 ? static_cast<double>(z[jn]) - static_cast<double>(z[s]) : 0.0;
         double d2 = px * px + py * py;
         if constexpr (DIM == 3) d2 = d2 + pz * pz;
-        double rho2 = seg_max_nonneg<SL>(li < N ? d2 : 0.0);
+        // the largest squared distance, NaN ignored (as the oracle's
+        // "d2 > rho2" scan and the single-wave engine's fmax do)
+        double rho2 = seg_max_nonneg<SL>(li < N && d2 == d2 ? d2 : 0.0);
         if (!(rho2 > 0.0)) rho2 = 1.0;
         const double rho = sqrt(rho2);
         rho2_out = rho2;

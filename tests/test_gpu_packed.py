@@ -125,3 +125,34 @@ def test_packed_small_batches_write_nothing_past_the_last_stencil(op, n, P, Q, n
         for c in range(plan.nops):
             np.testing.assert_array_equal(w[c][:nst * n], wref[c][:nst * n])
             assert np.isnan(w[c][nst * n:]).all()
+
+
+@pytest.mark.parametrize("op,n,P,Q,nrhs", [PACKED[0], PACKED[2], PACKED[4]])
+def test_packed_exact_ties_and_nan_stencils(op, n, P, Q, nrhs):
+    """An unjittered grid gives exactly tied pivot candidates (equal
+    distances, equal PHS values): the first row among them must win, as in
+    the oracle. A NaN coordinate makes every candidate of its stencils NaN:
+    the first active row is taken and info is the step, as in the oracle."""
+    m = 20
+    x, y = R.jittered_grid(2, m, 0.0, seed=1)
+    ja = R.knn_stencils(n, x, y)
+    x = x.copy()
+    x[123] = np.nan
+    nst = ja.size // n
+    cen = _centres(1.0 / m) if op == "interp" else None
+    wref, iref = _oracle(op, n, P, Q, x, y, ja, cen)
+    nanst = np.any(ja.reshape(-1, n) == 123, axis=1)
+    assert nanst.any() and (~nanst).sum() > 300
+    plan, packed = _plan(op, n, P, Q, nrhs)
+    plan.variant = [v for v in range(plan.num_variants) if v not in packed][0]
+    w0, i0, _ = _run(plan, nst, x, y, ja, cen)
+    np.testing.assert_array_equal(i0[:nst], iref)
+    for vp in packed:
+        plan.variant = vp
+        w, info, _ = _run(plan, nst, x, y, ja, cen)
+        print(f"{op} n={n} {plan.variant_name()}: {nanst.sum()} stencils with a NaN node, "
+              f"info {np.unique(info[:nst][nanst]).tolist()}")
+        np.testing.assert_array_equal(info[:nst], iref)
+        for c in range(plan.nops):
+            np.testing.assert_array_equal(w[c], wref[c])
+            np.testing.assert_array_equal(w[c], w0[c])
