@@ -424,3 +424,36 @@ def test_solve_dp_demo_system():
     np.testing.assert_array_equal(Bd.cpu().numpy().T, Xr)
     # interpolation at the stencil's own centre (0, 0) reproduces node 0
     np.testing.assert_allclose(Xr[:9, 0], np.eye(9)[0], atol=1e-12)
+
+
+@pytest.mark.parametrize("n,P,Q,prec", [(35, 3, 5, "fp64"), (56, 4, 7, "fp64"),
+                                        (35, 3, 5, "mixed"), (56, 4, 7, "mixed")])
+def test_exact_ties_and_nan_node_match_oracle(n, P, Q, prec):
+    """An unjittered grid (exactly tied pivot candidates: equal distances
+    give equal PHS values) and one NaN coordinate. FP64: every variant
+    bit-identical to the oracle, info included (NaN never wins the pivot;
+    with no number among the candidates the first active row is taken and
+    info names the step). Mixed: the stencils that hold the NaN node fall
+    back to the FP64 solve (status flag) and report the oracle's info; the
+    others stay within the mixed bar."""
+    m = 24
+    x, y = R.jittered_grid(2, m, 0.0, seed=1)
+    ja = R.knn_stencils(n, x, y)
+    x = x.copy()
+    x[m * (m // 2) + m // 2] = np.nan
+    wref, iref = O.assemble(2, n, P, Q, x, y, ja)
+    nanst = np.any(ja.reshape(-1, n) == m * (m // 2) + m // 2, axis=1)
+    assert nanst.any() and (iref[nanst] != 0).all() and (iref[~nanst] == 0).all()
+    plan = R.Plan(2, n, P, Q, "diff", precision=prec)
+    for v in range(plan.num_variants):
+        plan.variant = v
+        w, info, st = _run(plan, x, y, ja)
+        print(f"{prec} n={n} variant {v} ({plan.variant_name()}): {nanst.sum()} NaN stencils, "
+              f"info {np.unique(info[nanst]).tolist()}")
+        np.testing.assert_array_equal(info, iref)
+        if prec == "fp64":
+            for a, b in zip(w, wref):
+                np.testing.assert_array_equal(a, b)
+        else:
+            assert ((st[nanst] & 4) != 0).all()
+            assert _relerr(w, wref, n)[~nanst].max() <= MIXED_TOL
