@@ -25,6 +25,7 @@
 #include "rbf_assembly.h"
 #include "rbf_group.h"
 #include "rbf_wave.h"
+#include "rbf_wavepack.h"
 #include "rbf_tile.h"
 
 namespace {
@@ -67,6 +68,21 @@ k_list_group(int count, const int* __restrict__ order, const int* __restrict__ i
     const int s = order[i];
     G::run(s, static_cast<size_t>(ia[s]), x, y, z, ja, out, lds, info);
     if (status && threadIdx.x == 0) status[s] = 0;
+}
+
+// Several stencils per wave (16- or 32-lane segments, rbf_wavepack.h) from
+// the class list, one wave per workgroup.
+template <class E, int OCC>
+__global__ void __launch_bounds__(64, OCC)
+k_list_pack(int count, const int* __restrict__ order, const int* __restrict__ ia,
+            const double* __restrict__ x, const double* __restrict__ y,
+            const double* __restrict__ z, const int* __restrict__ ja, Out3 out,
+            int* __restrict__ info, uint8_t* __restrict__ status) {
+    __shared__ __align__(16) double lds[E::lds_doubles];
+    const int nwaves = (count + E::S - 1) / E::S;
+    const int b = xcd_block(nwaves);
+    if (b >= nwaves) return;
+    E::run_list(b * E::S, count, order, ia, x, y, z, ja, out, lds, info, status);
 }
 
 // ---- the persistent kernel -------------------------------------------------
@@ -171,6 +187,14 @@ hipError_t launch_list_wave(const ListArgs& a) {
                        a.count, a.order, a.ia, a.x, a.y, a.z, a.ja, a.out, a.info, a.status);
     return hipGetLastError();
 }
+template <class E, int OCC>
+hipError_t launch_list_pack(const ListArgs& a) {
+    if (a.count == 0) return hipSuccess;
+    const int nwaves = (a.count + E::S - 1) / E::S;
+    hipLaunchKernelGGL((k_list_pack<E, OCC>), dim3(xcd_grid(nwaves)), dim3(64), 0, a.stream,
+                       a.count, a.order, a.ia, a.x, a.y, a.z, a.ja, a.out, a.info, a.status);
+    return hipGetLastError();
+}
 template <class G, int THREADS, int OCC = 1>
 hipError_t launch_list_group(const ListArgs& a) {
     if (a.count == 0) return hipSuccess;
@@ -183,20 +207,49 @@ struct ListVariant {
     const char* name;
     ListFn fn;
 };
+constexpr int kMaxClassVariants = 6;
 struct BatchClass {
     int n, P, Q, nvariants, def;  // def: the variant used until autotuned
-    ListVariant v[3];
+    ListVariant v[kMaxClassVariants];
 };
 
+// The shapes a class's list can run in: stencils per wave (packed 16- or
+// 32-lane segments where the system fits 32 lanes), waves per workgroup
+// (1, 2 or 4 for one stencil per wave), and the wave LU's tuning. The
+// autotuner times each on the class's own stencils.
 template <int N, int P, int Q>
 constexpr BatchClass wave_class() {
     using T0 = detail::WaveAssembly<2, N, P, Q, 3, false, detail::LuTune<4, 2, 1, 0, 1>>;
     using T1 = detail::WaveAssembly<2, N, P, Q, 3, false, detail::LuTune<2, 1>>;
     using T2 = detail::WaveAssembly<2, N, P, Q, 3, false, detail::LuTune<0, 2, 1, 1, 1>>;
-    return {N, P, Q, 3, 0,
-            {{"bc=third pv=dpp-bcast pin fill=pairs", &launch_list_wave<T0, 1>},
-             {"bc=half pv=dpp", &launch_list_wave<T1, 1>},
-             {"bc=readlane pv=dpp-bcast pin fill=pairs bs=lds", &launch_list_wave<T2, 1>}}};
+    constexpr int NT = N + npoly(P);
+    if constexpr (NT <= 32) {
+        constexpr int SL = NT <= 16 ? 16 : 32;
+        constexpr int OCC2 = SL == 16 ? 4 : 3;
+        using PK1 = detail::WavePack<2, N, P, Q, 3, false, SL, 0, 1>;
+        using PK0 = detail::WavePack<2, N, P, Q, 3, false, SL, 0, 0>;
+        return {N, P, Q, 6, 0,
+                {{SL == 16 ? "packed: 4 stencils per wave bc=newbcast occ=4"
+                           : "packed: 2 stencils per wave bc=newbcast occ=3",
+                  &launch_list_pack<PK1, OCC2>},
+                 {SL == 16 ? "packed: 4 stencils per wave bc=newbcast"
+                           : "packed: 2 stencils per wave bc=newbcast",
+                  &launch_list_pack<PK1, 1>},
+                 {SL == 16 ? "packed: 4 stencils per wave" : "packed: 2 stencils per wave",
+                  &launch_list_pack<PK0, 1>},
+                 {"bc=third pv=dpp-bcast pin fill=pairs", &launch_list_wave<T0, 1>},
+                 {"bc=half pv=dpp", &launch_list_wave<T1, 1>},
+                 {"bc=readlane pv=dpp-bcast pin fill=pairs bs=lds", &launch_list_wave<T2, 1>}}};
+    } else {
+        return {N, P, Q, 5, 0,
+                {{"bc=third pv=dpp-bcast pin fill=pairs", &launch_list_wave<T0, 1>},
+                 {"bc=third pv=dpp-bcast pin fill=pairs, 2 waves per workgroup",
+                  &launch_list_wave<T0, 2>},
+                 {"bc=third pv=dpp-bcast pin fill=pairs, 4 waves per workgroup",
+                  &launch_list_wave<T0, 4>},
+                 {"bc=half pv=dpp", &launch_list_wave<T1, 1>},
+                 {"bc=readlane pv=dpp-bcast pin fill=pairs bs=lds", &launch_list_wave<T2, 1>}}};
+    }
 }
 
 const BatchClass kClasses[kMaxClasses] = {
@@ -205,16 +258,18 @@ const BatchClass kClasses[kMaxClasses] = {
     wave_class<35, 3, 5>(),
     {56, 4, 7, 2, 0,
      {{"blocked lu fp64 mfma", &launch_list_group<TP56, TP56::threads, TP56::occupancy>},
-      {"workgroup nw=2", &launch_list_group<GP56, 128>},
-      {nullptr, nullptr}}},
+      {"workgroup nw=2", &launch_list_group<GP56, 128>}}},
 };
 
 }  // namespace
 
 struct rbfa_batch {
     int nclasses = 0;
-    int cls[kMaxClasses];      // registry index of each user class
-    int variant[kMaxClasses];  // chosen list-kernel variant per user class
+    // classes are kept sorted by size (slot u); user_slot maps the caller's
+    // class number (its order at create) to its slot
+    int cls[kMaxClasses];        // registry index of each slot's class
+    int variant[kMaxClasses];    // chosen list-kernel variant per slot
+    int user_slot[kMaxClasses];  // slot of the caller's class i
     int mode = 0;
     bool bound = false;
     int32_t nstencils = 0;
@@ -329,6 +384,7 @@ RBFA_API int32_t rbfa_batch_create(int32_t nclasses, const rbfa_plan_desc* class
     for (int u = 0; u < nclasses; ++u) {
         b->cls[u] = cls[idx[u]];
         b->variant[u] = kClasses[b->cls[u]].def;
+        b->user_slot[idx[u]] = u;
     }
     int per_cu = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_persistent, kPersistentThreads,
@@ -414,9 +470,9 @@ RBFA_API int32_t rbfa_batch_bind(rbfa_batch* b, int32_t nstencils, const int32_t
     return RBFA_OK;
 }
 
-RBFA_API int32_t rbfa_batch_class_count(const rbfa_batch* b, int32_t u) {
-    if (!b || u < 0 || u >= b->nclasses) return -1;
-    return b->count[u];
+RBFA_API int32_t rbfa_batch_class_count(const rbfa_batch* b, int32_t cls) {
+    if (!b || cls < 0 || cls >= b->nclasses) return -1;
+    return b->count[b->user_slot[cls]];
 }
 
 RBFA_API int32_t rbfa_batch_set_mode(rbfa_batch* b, int32_t mode) {
@@ -493,9 +549,37 @@ RBFA_API int32_t rbfa_batch_autotune(rbfa_batch* b, void* stream, int32_t reps, 
     return rc;
 }
 
-RBFA_API const char* rbfa_batch_class_variant(const rbfa_batch* b, int32_t u) {
-    if (!b || u < 0 || u >= b->nclasses) return nullptr;
-    return kClasses[b->cls[u]].v[b->variant[u]].name;
+// Classes are numbered as the caller listed them at create; internally
+// they are kept sorted by size.
+static int batch_slot(const rbfa_batch* b, int32_t cls) {
+    if (!b || cls < 0 || cls >= b->nclasses) return -1;
+    return b->user_slot[cls];
+}
+
+RBFA_API const char* rbfa_batch_class_variant(const rbfa_batch* b, int32_t cls) {
+    const int u = batch_slot(b, cls);
+    return u < 0 ? nullptr : kClasses[b->cls[u]].v[b->variant[u]].name;
+}
+
+RBFA_API int32_t rbfa_batch_class_num_variants(const rbfa_batch* b, int32_t cls) {
+    const int u = batch_slot(b, cls);
+    return u < 0 ? -1 : kClasses[b->cls[u]].nvariants;
+}
+
+RBFA_API const char* rbfa_batch_class_variant_name(const rbfa_batch* b, int32_t cls, int32_t v) {
+    const int u = batch_slot(b, cls);
+    if (u < 0 || v < 0 || v >= kClasses[b->cls[u]].nvariants) return nullptr;
+    return kClasses[b->cls[u]].v[v].name;
+}
+
+RBFA_API int32_t rbfa_batch_set_class_variant(rbfa_batch* b, int32_t cls, int32_t v) {
+    const int u = batch_slot(b, cls);
+    if (u < 0 || v < 0 || v >= kClasses[b->cls[u]].nvariants) return RBFA_ERR_ARG;
+    if (b->variant[u] != v) {
+        b->variant[u] = v;
+        drop_graphs(b);  // the next launch captures the pipeline again
+    }
+    return RBFA_OK;
 }
 
 }  // extern "C"

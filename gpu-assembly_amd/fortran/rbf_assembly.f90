@@ -51,6 +51,8 @@ module rbf_assembly
   ! uneven batches
   public :: rbfa_batch_create, rbfa_batch_destroy, rbfa_batch_bind, rbfa_batch_class_count
   public :: rbfa_batch_class_variant, rbfa_batch_set_mode, rbfa_batch_get_mode
+  public :: rbfa_batch_class_num_variants, rbfa_batch_class_variant_name
+  public :: rbfa_batch_set_class_variant
   public :: rbfa_batch_launch, rbfa_batch_autotune
   ! caller-filled systems
   public :: rbfa_gesv_batched, rbfa_solve_dp
@@ -295,6 +297,24 @@ module rbf_assembly
       import :: c_int32_t, c_ptr
       type(c_ptr), value :: batch
       integer(c_int32_t), value :: cls
+    end function
+
+    integer(c_int32_t) function rbfa_batch_class_num_variants(batch, cls) bind(c)
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int32_t), value :: cls
+    end function
+
+    type(c_ptr) function rbfa_batch_class_variant_name(batch, cls, v) bind(c)
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int32_t), value :: cls, v
+    end function
+
+    integer(c_int32_t) function rbfa_batch_set_class_variant(batch, cls, v) bind(c)
+      import :: c_int32_t, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int32_t), value :: cls, v
     end function
 
     integer(c_int32_t) function rbfa_batch_set_mode(batch, mode) bind(c)

@@ -93,6 +93,9 @@ def lib():
         "rbfa_batch_bind": (i32, [P, i32, P, P, P, P, P, P, P, P, P, P]),
         "rbfa_batch_class_count": (i32, [P, i32]),
         "rbfa_batch_class_variant": (ctypes.c_char_p, [P, i32]),
+        "rbfa_batch_class_num_variants": (i32, [P, i32]),
+        "rbfa_batch_class_variant_name": (ctypes.c_char_p, [P, i32, i32]),
+        "rbfa_batch_set_class_variant": (i32, [P, i32, i32]),
         "rbfa_batch_set_mode": (i32, [P, i32]),
         "rbfa_batch_get_mode": (i32, [P]),
         "rbfa_batch_launch": (i32, [P, P]),
@@ -408,6 +411,17 @@ class Batch:
 
     def class_variant(self, c):
         return lib().rbfa_batch_class_variant(self._h, int(c)).decode()
+
+    def class_num_variants(self, c):
+        return lib().rbfa_batch_class_num_variants(self._h, int(c))
+
+    def class_variant_name(self, c, v):
+        r = lib().rbfa_batch_class_variant_name(self._h, int(c), int(v))
+        return None if r is None else r.decode()
+
+    def set_class_variant(self, c, v):
+        _check(lib().rbfa_batch_set_class_variant(self._h, int(c), int(v)),
+               "rbfa_batch_set_class_variant")
 
     @property
     def mode(self):

@@ -301,7 +301,15 @@ RBFA_API int32_t rbfa_plan_autotune(rbfa_plan* plan, int32_t nstencils, const do
  * RBFA_ERR_UNSUPPORTED). autotune picks each class's kernel variant and
  * the faster enabled mode (ms_out[2] = mode 0, mode 1 milliseconds; -1 for
  * a mode that is switched off). Calls that overlap on one batch race on
- * its work-queue counters. */
+ * its work-queue counters.
+ *
+ * Each class (numbered in the order of `classes`) lists the shapes its
+ * list launch can take: several stencils per wave (16- or 32-lane
+ * segments, where the system has at most 32 unknowns), one stencil per
+ * wave with 1, 2 or 4 waves per workgroup, a blocked-LU workgroup per
+ * stencil above 64 unknowns. class_num_variants / class_variant_name list
+ * them, set_class_variant picks one (autotune picks the fastest);
+ * class_variant names the current one. All give the same weights. */
 typedef struct rbfa_batch rbfa_batch;
 RBFA_API int32_t rbfa_batch_create(int32_t nclasses, const rbfa_plan_desc* classes,
                                    rbfa_batch** batch);
@@ -312,6 +320,10 @@ RBFA_API int32_t rbfa_batch_bind(rbfa_batch* batch, int32_t nstencils, const int
                                  int32_t* info, uint8_t* status, void* stream);
 RBFA_API int32_t rbfa_batch_class_count(const rbfa_batch* batch, int32_t cls);
 RBFA_API const char* rbfa_batch_class_variant(const rbfa_batch* batch, int32_t cls);
+RBFA_API int32_t rbfa_batch_class_num_variants(const rbfa_batch* batch, int32_t cls);
+RBFA_API const char* rbfa_batch_class_variant_name(const rbfa_batch* batch, int32_t cls,
+                                                   int32_t v);
+RBFA_API int32_t rbfa_batch_set_class_variant(rbfa_batch* batch, int32_t cls, int32_t v);
 RBFA_API int32_t rbfa_batch_set_mode(rbfa_batch* batch, int32_t mode);
 RBFA_API int32_t rbfa_batch_get_mode(const rbfa_batch* batch);
 RBFA_API int32_t rbfa_batch_launch(rbfa_batch* batch, void* stream);

@@ -444,25 +444,22 @@ struct WavePack {
         return info;
     }
 
-    // Stencils s0 .. s0 + S - 1 (one per segment; segments past the last
-    // stencil repeat it and store nothing), fixed-width rows of N.
+    // Stencil s of this lane's segment, whose CSR row starts at base; a
+    // segment with valid false repeats a real stencil and stores nothing.
     template <class Out, typename C>
-    __device__ __forceinline__ static void run(const int s0, const int nstencils,
-                                               const C* __restrict__ x, const C* __restrict__ y,
-                                               nodeduce<const C*> __restrict__ z,
-                                               const int* __restrict__ ja, Out out,
-                                               nodeduce<const C*> __restrict__ cx,
-                                               nodeduce<const C*> __restrict__ cy,
-                                               nodeduce<const C*> __restrict__ cz, double* lds,
-                                               int* __restrict__ info, uint8_t* __restrict__ status) {
+    __device__ __forceinline__ static void run_one(const int s, const size_t base, const bool valid,
+                                                   const C* __restrict__ x, const C* __restrict__ y,
+                                                   nodeduce<const C*> __restrict__ z,
+                                                   const int* __restrict__ ja, Out out,
+                                                   nodeduce<const C*> __restrict__ cx,
+                                                   nodeduce<const C*> __restrict__ cy,
+                                                   nodeduce<const C*> __restrict__ cz, double* lds,
+                                                   int* __restrict__ info,
+                                                   uint8_t* __restrict__ status) {
 #pragma clang fp contract(off)
         const int lane = lane_id();
         const int seg = lane / SL;
         const int li = lane & (SL - 1);
-        const int sw = s0 + seg;
-        const bool valid = sw < nstencils;
-        const int s = valid ? sw : nstencils - 1;
-        const size_t base = static_cast<size_t>(s) * N;
         double* segl = lds + seg * seg_doubles;
         double row[NC];
         double rho2;
@@ -484,6 +481,43 @@ struct WavePack {
             if (info) info[s] = inf;
             if (status) status[s] = 0;
         }
+    }
+
+    // Stencils s0 .. s0 + S - 1 (one per segment; segments past the last
+    // stencil repeat it and store nothing), fixed-width rows of N.
+    template <class Out, typename C>
+    __device__ __forceinline__ static void run(const int s0, const int nstencils,
+                                               const C* __restrict__ x, const C* __restrict__ y,
+                                               nodeduce<const C*> __restrict__ z,
+                                               const int* __restrict__ ja, Out out,
+                                               nodeduce<const C*> __restrict__ cx,
+                                               nodeduce<const C*> __restrict__ cy,
+                                               nodeduce<const C*> __restrict__ cz, double* lds,
+                                               int* __restrict__ info, uint8_t* __restrict__ status) {
+        const int sw = s0 + lane_id() / SL;
+        const bool valid = sw < nstencils;
+        const int s = valid ? sw : nstencils - 1;
+        run_one(s, static_cast<size_t>(s) * N, valid, x, y, z, ja, out, cx, cy, cz, lds, info,
+                status);
+    }
+
+    // Entries i0 .. i0 + S - 1 of a stencil list (order[i] names the
+    // stencil, ia[s] its CSR row start; entries past count repeat the last
+    // and store nothing): the per-class lists of a ragged batch.
+    template <class Out, typename C>
+    __device__ __forceinline__ static void run_list(const int i0, const int count,
+                                                    const int* __restrict__ order,
+                                                    const int* __restrict__ ia,
+                                                    const C* __restrict__ x, const C* __restrict__ y,
+                                                    nodeduce<const C*> __restrict__ z,
+                                                    const int* __restrict__ ja, Out out,
+                                                    double* lds, int* __restrict__ info,
+                                                    uint8_t* __restrict__ status) {
+        const int i = i0 + lane_id() / SL;
+        const bool valid = i < count;
+        const int s = order[valid ? i : count - 1];
+        run_one(s, static_cast<size_t>(ia[s]), valid, x, y, z, ja, out, nullptr, nullptr, nullptr,
+                lds, info, status);
     }
 };
 

@@ -288,6 +288,47 @@ def test_uneven_batch_autotune():
     assert all(b.class_variant(c) for c in range(4))
 
 
+def test_uneven_batch_every_class_shape_matches_oracle():
+    """Every shape a class's list launch can take (stencils per wave, waves
+    per workgroup, engine tuning) gives the oracle's weights bit for bit;
+    the classes are numbered in the caller's order, here given largest
+    first."""
+    import torch
+    from rbf_amd import workloads
+    W = _disk_case(6000, 200, 24)
+    ia, ja, x, y = W["ia"], W["ja"], W["x"], W["y"]
+    nnz = int(ia[-1])
+    wref = [np.zeros(nnz) for _ in range(3)]
+    iref = np.zeros(x.size, np.int32)
+    for (n, P, Q) in workloads.DISK_CLASSES:
+        rows = np.nonzero(W["sizes"] == n)[0].astype(np.int32)
+        O.assemble_rows(2, n, P, Q, rows, ia, x, y, ja, wref, iref)
+    classes = list(reversed(workloads.DISK_CLASSES))
+    b = R.Batch(classes)
+    w = [torch.zeros(nnz, dtype=torch.float64, device="cuda:0") for _ in range(3)]
+    info = torch.full((x.size,), -1, dtype=torch.int32, device="cuda:0")
+    b.bind(ia, _dev(x), _dev(y), _dev(ia), _dev(ja), w, info=info)
+    for c, (n, _, _) in enumerate(classes):
+        assert b.class_count(c) == int((W["sizes"] == n).sum())
+        nv = b.class_num_variants(c)
+        assert nv >= 2 and b.class_variant_name(c, nv) is None
+        with pytest.raises(R.RbfError):
+            b.set_class_variant(c, nv)
+        for v in range(nv):
+            b.set_class_variant(c, v)
+            assert b.class_variant(c) == b.class_variant_name(c, v)
+            for t in w:
+                t.zero_()
+            info.fill_(-1)
+            b.launch()
+            torch.cuda.synchronize()
+            print(f"class n={n} variant {v}: {b.class_variant(c)}")
+            assert (info.cpu().numpy() == 0).all()
+            for a, r in zip(w, wref):
+                np.testing.assert_array_equal(a.cpu().numpy(), r)
+        b.set_class_variant(c, 0)
+
+
 def test_uneven_batch_failed_bind_leaves_it_unbound():
     """A bind that fails (a row length no class has) leaves the batch
     unbound: launch and autotune refuse it until a bind succeeds."""
