@@ -205,11 +205,6 @@ struct GroupVariants {
         {"workgroup nw=2 bc=lds pv=dpp", &launch_group_diff<DIM, N, P, Q>}};
 };
 template <int DIM, int N, int P, int Q>
-struct GroupMixVariants {
-    static constexpr Variant list[1] = {
-        {"workgroup nw=2 fp32 lu + fp64 refinement", &launch_group_mixed<DIM, N, P, Q>}};
-};
-template <int DIM, int N, int P, int Q>
 struct NsListMixVariants {
     static constexpr Variant list[3] = {
         {"null space: fp64 poly lu + mfma schur, fp32 lu 16 symmetric steps, blocked-lu fallback",
@@ -218,12 +213,6 @@ struct NsListMixVariants {
          &launch_ns_list<DIM, N, P, Q, 64, 1>},
         {"workgroup nw=2 fp32 lu + fp64 refinement", &launch_group_mixed<DIM, N, P, Q>}};
 };
-template <int DIM, int N, int P, int Q>
-constexpr Shape diffgroupmix() {
-    constexpr int M = npoly_dim(DIM, P);
-    return {DIM, N, P, Q, RBFA_OPS_DIFF, 0, RBFA_PREC_MIXED, N + M, DIM + 1,
-            GroupMixVariants<DIM, N, P, Q>::list, 1, true};
-}
 template <int DIM, int N, int P, int Q>
 constexpr Shape diffnsmix() {
     constexpr int M = npoly_dim(DIM, P);
@@ -284,7 +273,7 @@ const Shape kShapes[] = {
     diffmix<2, 35, 3, 3>(), diffmix<2, 21, 2, 5>(), diffmix<2, 13, 1, 3>(),
     // systems larger than a wavefront: one two-wave workgroup per stencil
     diffgroup<2, 56, 4, 7>(), diffgroup<3, 80, 3, 3>(),
-    diffnsmix<2, 56, 4, 7>(), diffgroupmix<3, 80, 3, 3>(),
+    diffnsmix<2, 56, 4, 7>(), diffnsmix<3, 80, 3, 3>(),
 };
 
 }  // namespace

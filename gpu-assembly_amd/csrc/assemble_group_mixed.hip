@@ -68,8 +68,8 @@ hipError_t launch_group_mixed(const LaunchArgs& a) {
 }
 
 // Null-space mixed form of systems larger than a wavefront: one wave per
-// stencil (n <= 64 nodes, n - m <= 64 free nodes), stencils that do not
-// converge listed for the blocked-LU FP64 pass.
+// stencil (n <= 128 nodes, two per lane above 64; n - m <= 64 free nodes),
+// stencils that do not converge listed for the blocked-LU FP64 pass.
 template <class E, int OCC>
 __global__ void __launch_bounds__(64, OCC)
 k_assemble_ns_list(int nstencils, const double* __restrict__ x, const double* __restrict__ y,
@@ -104,6 +104,8 @@ hipError_t launch_ns_list(const LaunchArgs& a) {
 
 template hipError_t launch_ns_list<2, 56, 4, 7, 16, 1>(const LaunchArgs&);
 template hipError_t launch_ns_list<2, 56, 4, 7, 64, 1>(const LaunchArgs&);
+template hipError_t launch_ns_list<3, 80, 3, 3, 16, 1>(const LaunchArgs&);
+template hipError_t launch_ns_list<3, 80, 3, 3, 64, 1>(const LaunchArgs&);
 template hipError_t launch_group_mixed<2, 56, 4, 7>(const LaunchArgs&);
 template hipError_t launch_group_mixed<3, 80, 3, 3>(const LaunchArgs&);
 

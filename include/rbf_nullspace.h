@@ -38,6 +38,11 @@
 // and flagged RBFA_STAT_FALLBACK; its weights and info are then those of
 // the FP64 form. Systems of up to 64 unknowns take the wave LU in the same
 // wave; larger ones are listed (fb_count, fb_list) for the blocked LU.
+//
+// Stencils of more than 64 nodes (up to 128; at most 64 free nodes) keep
+// two nodes per lane, node l and node 64 + l, through the node-wise phases
+// (gather, A, the kernel entries against the pivot nodes, G, t, y); every
+// phase from the Schur complement on works on free ranks, one per lane.
 #ifndef RBF_NULLSPACE_H
 #define RBF_NULLSPACE_H
 
@@ -69,7 +74,8 @@ struct NullSpaceMixed {
     static constexpr int NTILE = T * (T + 1) / 2;  // upper-triangle tiles (S is symmetric)
     static constexpr int K2 = 2 * M;  // inner dimension of the Schur update
     static constexpr int KS = (K2 + 3) / 4;  // MFMA k-steps
-    static_assert(N <= 64 && F >= 1, "one node per lane");
+    static constexpr int NSETS = N > 64 ? 2 : 1;  // nodes per lane
+    static_assert(N <= 128 && F >= 1 && F <= 64, "two nodes per lane at most, one free rank per lane");
     static_assert(Q >= 3 && Q % 2 == 1, "odd PHS exponent >= 3");
     // Fallback: systems of up to 64 unknowns are solved again in the same
     // wave by the FP64 wave LU; larger ones are appended to a list that a
@@ -267,6 +273,292 @@ struct NullSpaceMixed {
         }
     }
 
+    // Phases A and D at two nodes per lane (node l and node 64 + l; N > 64):
+    // gather, shift and scale; the polynomial block's LU with partial
+    // pivoting over both node sets (the largest high word wins, the lowest
+    // node among equal ones); the kernel entries against the pivot nodes;
+    // G; the particular solution and H (rank-wise); t and the reduced
+    // right-hand sides y. Free nodes take ranks in node order. Leaves the
+    // LDS as the one-node-per-lane phases do and returns the scale and the
+    // rank-deficiency flag.
+    __device__ __forceinline__ static void node_phases_two(
+        const int s, const int* __restrict__ ja_row, const double* __restrict__ x,
+        const double* __restrict__ y, const double* __restrict__ z, const int lane, double* lds,
+        double& irho, double& irho2, bool& bad) {
+#pragma clang fp contract(off)
+        constexpr int N1 = N - 64;  // nodes of the second set
+        constexpr unsigned long long kRows1 = N1 >= 64 ? ~0ull : ((1ull << N1) - 1);
+        double* qf = lds + o_qf;
+        double* yb = lds + o_y;
+        double* gb = lds + o_g;
+        double* wpb = lds + o_wp;
+        double* rub = lds + o_ru;
+        int* fidx = reinterpret_cast<int*>(lds + o_int);
+        int* pvn = fidx + F;
+        double* lb = lds + o_l;
+        double* app = lds + o_app;
+        double* tb = lds + o_t;
+        double* hb = lds + o_h;
+        const bool has1 = lane_in(kRows1);
+
+        // gather, shift, scale; the radius from the largest squared distance
+        // by its high word, as with one node per lane
+        double qx[2], qy[2], qz[2], d2[2];
+        unsigned dkey[2];
+        {
+            const int jn0 = ja_row[lane];
+            const int jn1 = has1 ? ja_row[64 + lane] : s;
+            const int jn[2] = {jn0, jn1};
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                qx[k] = x[jn[k]] - x[s];
+                qy[k] = y[jn[k]] - y[s];
+                qz[k] = DIM == 3 ? z[jn[k]] - z[s] : 0.0;
+                d2[k] = qx[k] * qx[k] + qy[k] * qy[k];
+                if constexpr (DIM == 3) d2[k] = d2[k] + qz[k] * qz[k];
+            }
+        }
+        dkey[0] = static_cast<unsigned>(__double2hiint(d2[0]));
+        dkey[1] = has1 ? static_cast<unsigned>(__double2hiint(d2[1])) : 0u;
+        const unsigned dmax = wave_umax_bcast(dkey[0] > dkey[1] ? dkey[0] : dkey[1]);
+        const unsigned long long at0 = wave_ballot(dkey[0] == dmax);
+        double rho2 = at0 ? bcast(d2[0], __ffsll(at0) - 1)
+                          : bcast(d2[1], __ffsll(wave_ballot(dkey[1] == dmax)) - 1);
+        if (!(rho2 > 0.0)) rho2 = 1.0;
+        const double rho = sqrt(rho2);
+        irho = wave_uniform(1.0 / rho);
+        irho2 = irho * irho;
+        double mono[2][M];
+        double bk[2][R];
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            qx[k] = qx[k] * irho;
+            qy[k] = qy[k] * irho;
+            qz[k] = qz[k] * irho;
+            PolyBasis<P, DIM>{}(qx[k], qy[k], qz[k], mono[k], 1);
+            double r2 = qx[k] * qx[k] + qy[k] * qy[k];
+            if constexpr (DIM == 3) r2 = r2 + qz[k] * qz[k];
+            const double tq = rpow<Q - 2>(r2);
+            const double gq = static_cast<double>(Q) * tq;
+            bk[k][0] = static_cast<double>(Q * (Q + DIM - 2)) * tq;
+            bk[k][1] = -gq * qx[k];
+            bk[k][2] = -gq * qy[k];
+            if constexpr (DIM == 3) bk[k][3] = -gq * qz[k];
+        }
+
+        // A: LU with partial pivoting of P over both node sets
+        unsigned long long act[2] = {~0ull, kRows1};
+        int mypt[2] = {M, M};
+#pragma unroll
+        for (int t = 0; t < M; ++t) {
+            unsigned key[2];
+#pragma unroll
+            for (int k = 0; k < 2; ++k)
+                key[k] = lane_in(act[k])
+                             ? (static_cast<unsigned>(__double2hiint(mono[k][t])) & 0x7fffffffu)
+                             : 0u;
+            const unsigned mk = wave_umax_bcast(key[0] > key[1] ? key[0] : key[1]);
+            const unsigned long long c0 = act[0] & wave_ballot(key[0] == mk);
+            const int ps = c0 ? 0 : 1;  // set of the pivot node
+            const int p = __ffsll(c0 ? c0 : act[1] & wave_ballot(key[1] == mk)) - 1;
+            const double piv = bcast(ps ? mono[1][t] : mono[0][t], p);
+            if (!(piv != 0.0)) bad = true;
+            double rcp = __builtin_amdgcn_rcp(piv);  // two Newton steps from the seed
+            rcp = fma(rcp, fma(-piv, rcp, 1.0), rcp);
+            rcp = fma(rcp, fma(-piv, rcp, 1.0), rcp);
+            double pv[M];  // the pivot row right of column t
+#pragma unroll
+            for (int u = t + 1; u < M; ++u) pv[u] = bcast(ps ? mono[1][u] : mono[0][u], p);
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const unsigned long long pm = ps == k ? 1ull << p : 0ull;
+                if (lane_in(act[k] & ~pm)) {
+                    const double l = mono[k][t] * rcp;
+#pragma unroll
+                    for (int u = t + 1; u < M; ++u) mono[k][u] = fma(-l, pv[u], mono[k][u]);
+                    mono[k][t] = l;
+                }
+                mypt[k] = lane_in(pm) ? t : mypt[k];
+                act[k] &= ~pm;
+            }
+            if (lane == 0) rub[t] = rcp;
+        }
+        const bool isfree[2] = {lane_in(act[0]), lane_in(act[1])};
+        const int nfree0 = __popcll(act[0]);
+        const int rank[2] = {
+            static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                static_cast<unsigned>(act[0] >> 32),
+                __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(act[0]), 0u))),
+            nfree0 + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                         static_cast<unsigned>(act[1] >> 32),
+                         __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(act[1]), 0u)))};
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (mypt[k] < M) {
+#pragma unroll
+                for (int u = 0; u < M; ++u) lb[mypt[k] * M + u] = mono[k][u];
+                pvn[mypt[k]] = lane + 64 * k;
+                tb[mypt[k] * R] = qx[k];
+                tb[mypt[k] * R + 1] = qy[k];
+                if constexpr (DIM == 3) tb[mypt[k] * R + 2] = qz[k];
+            }
+        }
+        wave_lds_sync();
+        // kernel entries against the pivot nodes
+        double akp[2][M];
+#pragma unroll
+        for (int t = 0; t < M; ++t) {
+            const double bx = tb[t * R], by = tb[t * R + 1], bz = DIM == 3 ? tb[t * R + 2] : 0.0;
+#pragma unroll
+            for (int k = 0; k < 2; ++k) akp[k][t] = phi_pair(qx[k], qy[k], qz[k], bx, by, bz);
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (mypt[k] < M) {
+#pragma unroll
+                for (int t = 0; t < M; ++t) app[mypt[k] * M + t] = akp[k][t];
+            }
+        }
+        if (lane < TP - F) {
+#pragma unroll
+            for (int a = 0; a < DIM; ++a) qf[a * TP + F + lane] = 0.0;
+#pragma unroll
+            for (int t = 0; t < M; ++t) hb[(F + lane) * GS + t] = 0.0;
+        }
+        wave_lds_sync();
+        // G = L2 L1^-1, both sets
+#pragma unroll
+        for (int t = M - 2; t >= 0; --t) {
+            double acc[2] = {mono[0][t], mono[1][t]};
+#pragma unroll
+            for (int u = t + 1; u < M; ++u) {
+                const double l1 = lb[u * M + t];
+#pragma unroll
+                for (int k = 0; k < 2; ++k) acc[k] = fma(-mono[k][u], l1, acc[k]);
+            }
+            mono[0][t] = acc[0];
+            mono[1][t] = acc[1];
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (isfree[k]) {
+                qf[rank[k]] = qx[k];
+                qf[TP + rank[k]] = qy[k];
+                if constexpr (DIM == 3) qf[2 * TP + rank[k]] = qz[k];
+                fidx[rank[k]] = lane + 64 * k;
+#pragma unroll
+                for (int t = 0; t < M; ++t) {
+                    gb[rank[k] * GS + t] = mono[k][t];
+                    hb[rank[k] * GS + t] = akp[k][t];
+                }
+            }
+        }
+
+        // D: particular solution and H; t = b - A_:pi w_p at every node
+        particular_and_h(lane, lb, rub, wpb, gb, hb, app);
+        double tk[2][R];
+#pragma unroll
+        for (int c = 0; c < R; ++c) {
+            double acc[2] = {bk[0][c], bk[1][c]};
+#pragma unroll
+            for (int u = 0; u < M; ++u) {
+                const double wp = wpb[u * R + c];
+#pragma unroll
+                for (int k = 0; k < 2; ++k) acc[k] = fma(-akp[k][u], wp, acc[k]);
+            }
+            tk[0][c] = acc[0];
+            tk[1][c] = acc[1];
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (mypt[k] < M) {
+#pragma unroll
+                for (int c = 0; c < R; ++c) tb[mypt[k] * R + c] = tk[k][c];
+            }
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (isfree[k]) {
+#pragma unroll
+                for (int c = 0; c < R; ++c) {
+                    double acc = tk[k][c];
+#pragma unroll
+                    for (int u = 0; u < M; ++u) acc = fma(-mono[k][u], tb[u * R + c], acc);
+                    yb[rank[k] * R + c] = acc;
+                }
+            }
+        }
+    }
+
+    // The rank-wise part of phase D: lane c < R solves P_pi^T w = g_c
+    // (U^T z = g_c forward, L1^T w = z back) into wpb; G's padding rows
+    // then take the place of P's pivot rows, and H = A_Fpi - G A_pipi / 2
+    // is formed on FP64 MFMA: tile I of free ranks, the m pivot columns
+    // (C = A_Fpi from hb, A = -G/2, B = A_pipi).
+    __device__ __forceinline__ static void particular_and_h(const int lane, const double* lb,
+                                                            const double* rub, double* wpb,
+                                                            double* gb, double* hb,
+                                                            const double* app) {
+#pragma clang fp contract(off)
+        if (lane < R) {
+            using Mk = PolyDerivMasks<DIM, P>;
+            const uint64_t gm = lane == 0 ? Mk::lap : (lane == 1 ? Mk::dx : (lane == 2 ? Mk::dy : Mk::dz));
+            const double gv = lane == 0 ? 2.0 : 1.0;
+            double w[M];
+#pragma unroll
+            for (int t = 0; t < M; ++t) {
+                double acc = ((gm >> t) & 1u) ? gv : 0.0;
+#pragma unroll
+                for (int u = 0; u < t; ++u) acc = fma(-lb[u * M + t], w[u], acc);
+                w[t] = acc * rub[t];
+            }
+#pragma unroll
+            for (int t = M - 2; t >= 0; --t) {
+                double acc = w[t];
+#pragma unroll
+                for (int u = t + 1; u < M; ++u) acc = fma(-lb[u * M + t], w[u], acc);
+                w[t] = acc;
+            }
+#pragma unroll
+            for (int t = 0; t < M; ++t) wpb[t * R + lane] = w[t];
+        }
+        wave_lds_sync();  // P's pivot rows read: G's padding rows take their place
+        if (lane < TP - F) {
+#pragma unroll
+            for (int t = 0; t < M; ++t) gb[(F + lane) * GS + t] = 0.0;
+        }
+        wave_lds_sync();
+        {
+            const int li = lane & 15, lg = lane >> 4;
+            constexpr int KH = (M + 3) / 4;
+            constexpr int TH = (M + 15) / 16;  // column tiles of H (pivot columns)
+#pragma unroll
+            for (int I = 0; I < T; ++I) {
+#pragma unroll
+                for (int J = 0; J < TH; ++J) {
+                    const int col = 16 * J + li;
+                    ns_dbl4 c;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        c[v] = col < M ? hb[(16 * I + lg + 4 * v) * GS + col] : 0.0;
+#pragma unroll
+                    for (int k = 0; k < KH; ++k) {
+                        const int kk = 4 * k + lg;
+                        const double av = kk < M ? -0.5 * gb[(16 * I + li) * GS + kk] : 0.0;
+                        const double bv = (kk < M && col < M) ? app[kk * M + col] : 0.0;
+                        c = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, c, 0, 0, 0);
+                    }
+                    if (col < M) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) hb[(16 * I + lg + 4 * v) * GS + col] = c[v];
+                    }
+                }
+            }
+        }
+        wave_lds_sync();
+    }
+
     template <class Out>
     __device__ __forceinline__ static void run(const int s_arg, const size_t base_arg,
                                                const double* __restrict__ x,
@@ -297,202 +589,154 @@ struct NullSpaceMixed {
         double* hb = lds + o_h;
         double* sb = lds + o_s;
 
-        // ---- gather, shift, scale (as the FP64 path) ---------------------
-        const int jn = lane < N ? ja_row[lane] : s;
-        const double px = x[jn] - x[s];
-        const double py = y[jn] - y[s];
-        const double pz = DIM == 3 ? z[jn] - z[s] : 0.0;
-        double d2 = px * px + py * py;
-        if constexpr (DIM == 3) d2 = d2 + pz * pz;
-        // the stencil radius: the node with the largest squared distance by
-        // its high word (the scaling only needs a radius of the right size)
-        const unsigned dkey = lane < N ? static_cast<unsigned>(__double2hiint(d2)) : 0u;
-        const unsigned dmax = wave_umax_bcast(dkey);
-        double rho2 = bcast(d2, __ffsll(wave_ballot(dkey == dmax)) - 1);
-        if (!(rho2 > 0.0)) rho2 = 1.0;
-        const double rho = sqrt(rho2);
-        const double irho = wave_uniform(1.0 / rho);
-        const double irho2 = irho * irho;
-        const double qx = px * irho, qy = py * irho, qz = pz * irho;
-        double mono[M];
-        PolyBasis<P, DIM>{}(qx, qy, qz, mono, 1);
-        // right-hand sides of the kernel rows (the Laplacian and gradient of
-        // phi(|q - q_k|) at the origin)
-        double bk[R];
-        {
-            double r2 = qx * qx + qy * qy;
-            if constexpr (DIM == 3) r2 = r2 + qz * qz;
-            const double tq = rpow<Q - 2>(r2);
-            const double gq = static_cast<double>(Q) * tq;
-            bk[0] = static_cast<double>(Q * (Q + DIM - 2)) * tq;
-            bk[1] = -gq * qx;
-            bk[2] = -gq * qy;
-            if constexpr (DIM == 3) bk[3] = -gq * qz;
-        }
-
-        // ---- A: LU with partial pivoting of P, FP64 ------------------------
-        constexpr unsigned long long kRows = N >= 64 ? ~0ull : ((1ull << N) - 1);
-        unsigned long long act = kRows;
-        int mypt = M;  // pivot step of this lane's node (M: a free node)
+        double irho, irho2;
         bool bad = false;
-#pragma unroll
-        for (int t = 0; t < M; ++t) {
-            const double a = mono[t];
-            // pivot: the largest |a| among the rows still active, compared by
-            // its high word (lowest lane among equal high words: any of them
-            // serves the conditioning); NaN wins and sends the stencil to the
-            // FP64 solve through the checks below
-            const unsigned key =
-                lane_in(act) ? (static_cast<unsigned>(__double2hiint(a)) & 0x7fffffffu) : 0u;
-            const unsigned mk = wave_umax_bcast(key);
-            const int p = __ffsll(act & wave_ballot(key == mk)) - 1;
-            const double piv = bcast(a, p);
-            if (!(piv != 0.0)) bad = true;
-            double rcp = __builtin_amdgcn_rcp(piv);  // two Newton steps from the seed
-            rcp = fma(rcp, fma(-piv, rcp, 1.0), rcp);
-            rcp = fma(rcp, fma(-piv, rcp, 1.0), rcp);
-            const unsigned long long pm = 1ull << p;
-            if (lane_in(act & ~pm)) {
-                const double l = a * rcp;
-#pragma unroll
-                for (int u = t + 1; u < M; ++u) mono[u] = fma(-l, bcast(mono[u], p), mono[u]);
-                mono[t] = l;
+        if constexpr (NSETS == 2) {
+            node_phases_two(s, ja_row, x, y, z, lane, lds, irho, irho2, bad);
+        } else {
+            // ---- gather, shift, scale (as the FP64 path) ---------------------
+            const int jn = lane < N ? ja_row[lane] : s;
+            const double px = x[jn] - x[s];
+            const double py = y[jn] - y[s];
+            const double pz = DIM == 3 ? z[jn] - z[s] : 0.0;
+            double d2 = px * px + py * py;
+            if constexpr (DIM == 3) d2 = d2 + pz * pz;
+            // the stencil radius: the node with the largest squared distance by
+            // its high word (the scaling only needs a radius of the right size)
+            const unsigned dkey = lane < N ? static_cast<unsigned>(__double2hiint(d2)) : 0u;
+            const unsigned dmax = wave_umax_bcast(dkey);
+            double rho2 = bcast(d2, __ffsll(wave_ballot(dkey == dmax)) - 1);
+            if (!(rho2 > 0.0)) rho2 = 1.0;
+            const double rho = sqrt(rho2);
+            irho = wave_uniform(1.0 / rho);
+            irho2 = irho * irho;
+            const double qx = px * irho, qy = py * irho, qz = pz * irho;
+            double mono[M];
+            PolyBasis<P, DIM>{}(qx, qy, qz, mono, 1);
+            // right-hand sides of the kernel rows (the Laplacian and gradient of
+            // phi(|q - q_k|) at the origin)
+            double bk[R];
+            {
+                double r2 = qx * qx + qy * qy;
+                if constexpr (DIM == 3) r2 = r2 + qz * qz;
+                const double tq = rpow<Q - 2>(r2);
+                const double gq = static_cast<double>(Q) * tq;
+                bk[0] = static_cast<double>(Q * (Q + DIM - 2)) * tq;
+                bk[1] = -gq * qx;
+                bk[2] = -gq * qy;
+                if constexpr (DIM == 3) bk[3] = -gq * qz;
             }
-            mypt = lane_in(pm) ? t : mypt;
-            if (lane == 0) rub[t] = rcp;
-            act &= ~pm;
-        }
-        const unsigned long long freem = act;  // free nodes, in lane order
-        const bool isfree = lane_in(freem);
-        const int rank = __builtin_amdgcn_mbcnt_hi(
-            static_cast<unsigned>(freem >> 32),
-            __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(freem), 0u));
-        // pivot rows: L1 below the diagonal, U on and above it; the pivot
-        // nodes' points (in t_pi's place, which is written later)
-        if (mypt < M) {
-#pragma unroll
-            for (int u = 0; u < M; ++u) lb[mypt * M + u] = mono[u];
-            pvn[mypt] = lane;
-            tb[mypt * R] = qx;
-            tb[mypt * R + 1] = qy;
-            if constexpr (DIM == 3) tb[mypt * R + 2] = qz;
-        }
-        wave_lds_sync();
-        // kernel entries against the pivot nodes (the pivot lanes' values
-        // form A_pipi)
-        double akp[M];
-#pragma unroll
-        for (int t = 0; t < M; ++t)
-            akp[t] = phi_pair(qx, qy, qz, tb[t * R], tb[t * R + 1], DIM == 3 ? tb[t * R + 2] : 0.0);
-        if (mypt < M) {
-#pragma unroll
-            for (int t = 0; t < M; ++t) app[mypt * M + t] = akp[t];
-        }
-        // padding ranks F .. TP-1: zero points and H rows (G's below)
-        if (lane < TP - F) {
-#pragma unroll
-            for (int a = 0; a < DIM; ++a) qf[a * TP + F + lane] = 0.0;
-#pragma unroll
-            for (int t = 0; t < M; ++t) hb[(F + lane) * GS + t] = 0.0;
-        }
-        wave_lds_sync();
-        // G = L2 L1^-1: g L1 = l, from the last column (L1 read from LDS; the
-        // pivot lanes compute values nobody reads)
-#pragma unroll
-        for (int t = M - 2; t >= 0; --t) {
-            double acc = mono[t];
-#pragma unroll
-            for (int u = t + 1; u < M; ++u) acc = fma(-mono[u], lb[u * M + t], acc);
-            mono[t] = acc;
-        }
-        if (isfree) {
-            qf[rank] = qx;
-            qf[TP + rank] = qy;
-            if constexpr (DIM == 3) qf[2 * TP + rank] = qz;
-            fidx[rank] = lane;
-#pragma unroll
-            for (int t = 0; t < M; ++t) {
-                gb[rank * GS + t] = mono[t];
-                hb[rank * GS + t] = akp[t];
-            }
-        }
 
-        // ---- D: particular solution, t, y -----------------------------------
-        // lane c < R solves P_pi^T w = g_c: U^T z = g_c forward, L1^T w = z back
-        if (lane < R) {
-            using Mk = PolyDerivMasks<DIM, P>;
-            const uint64_t gm = lane == 0 ? Mk::lap : (lane == 1 ? Mk::dx : (lane == 2 ? Mk::dy : Mk::dz));
-            const double gv = lane == 0 ? 2.0 : 1.0;
-            double w[M];
-#pragma unroll
+            // ---- A: LU with partial pivoting of P, FP64 ------------------------
+            constexpr unsigned long long kRows = N >= 64 ? ~0ull : ((1ull << N) - 1);
+            unsigned long long act = kRows;
+            int mypt = M;  // pivot step of this lane's node (M: a free node)
+    #pragma unroll
             for (int t = 0; t < M; ++t) {
-                double acc = ((gm >> t) & 1u) ? gv : 0.0;
-#pragma unroll
-                for (int u = 0; u < t; ++u) acc = fma(-lb[u * M + t], w[u], acc);
-                w[t] = acc * rub[t];
+                const double a = mono[t];
+                // pivot: the largest |a| among the rows still active, compared by
+                // its high word (lowest lane among equal high words: any of them
+                // serves the conditioning); NaN wins and sends the stencil to the
+                // FP64 solve through the checks below
+                const unsigned key =
+                    lane_in(act) ? (static_cast<unsigned>(__double2hiint(a)) & 0x7fffffffu) : 0u;
+                const unsigned mk = wave_umax_bcast(key);
+                const int p = __ffsll(act & wave_ballot(key == mk)) - 1;
+                const double piv = bcast(a, p);
+                if (!(piv != 0.0)) bad = true;
+                double rcp = __builtin_amdgcn_rcp(piv);  // two Newton steps from the seed
+                rcp = fma(rcp, fma(-piv, rcp, 1.0), rcp);
+                rcp = fma(rcp, fma(-piv, rcp, 1.0), rcp);
+                const unsigned long long pm = 1ull << p;
+                if (lane_in(act & ~pm)) {
+                    const double l = a * rcp;
+    #pragma unroll
+                    for (int u = t + 1; u < M; ++u) mono[u] = fma(-l, bcast(mono[u], p), mono[u]);
+                    mono[t] = l;
+                }
+                mypt = lane_in(pm) ? t : mypt;
+                if (lane == 0) rub[t] = rcp;
+                act &= ~pm;
             }
-#pragma unroll
+            const unsigned long long freem = act;  // free nodes, in lane order
+            const bool isfree = lane_in(freem);
+            const int rank = __builtin_amdgcn_mbcnt_hi(
+                static_cast<unsigned>(freem >> 32),
+                __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(freem), 0u));
+            // pivot rows: L1 below the diagonal, U on and above it; the pivot
+            // nodes' points (in t_pi's place, which is written later)
+            if (mypt < M) {
+    #pragma unroll
+                for (int u = 0; u < M; ++u) lb[mypt * M + u] = mono[u];
+                pvn[mypt] = lane;
+                tb[mypt * R] = qx;
+                tb[mypt * R + 1] = qy;
+                if constexpr (DIM == 3) tb[mypt * R + 2] = qz;
+            }
+            wave_lds_sync();
+            // kernel entries against the pivot nodes (the pivot lanes' values
+            // form A_pipi)
+            double akp[M];
+    #pragma unroll
+            for (int t = 0; t < M; ++t)
+                akp[t] = phi_pair(qx, qy, qz, tb[t * R], tb[t * R + 1], DIM == 3 ? tb[t * R + 2] : 0.0);
+            if (mypt < M) {
+    #pragma unroll
+                for (int t = 0; t < M; ++t) app[mypt * M + t] = akp[t];
+            }
+            // padding ranks F .. TP-1: zero points and H rows (G's below)
+            if (lane < TP - F) {
+    #pragma unroll
+                for (int a = 0; a < DIM; ++a) qf[a * TP + F + lane] = 0.0;
+    #pragma unroll
+                for (int t = 0; t < M; ++t) hb[(F + lane) * GS + t] = 0.0;
+            }
+            wave_lds_sync();
+            // G = L2 L1^-1: g L1 = l, from the last column (L1 read from LDS; the
+            // pivot lanes compute values nobody reads)
+    #pragma unroll
             for (int t = M - 2; t >= 0; --t) {
-                double acc = w[t];
-#pragma unroll
-                for (int u = t + 1; u < M; ++u) acc = fma(-lb[u * M + t], w[u], acc);
-                w[t] = acc;
+                double acc = mono[t];
+    #pragma unroll
+                for (int u = t + 1; u < M; ++u) acc = fma(-mono[u], lb[u * M + t], acc);
+                mono[t] = acc;
             }
-#pragma unroll
-            for (int t = 0; t < M; ++t) wpb[t * R + lane] = w[t];
-        }
-        wave_lds_sync();  // P's pivot rows read: G's padding rows take their place
-        if (lane < TP - F) {
-#pragma unroll
-            for (int t = 0; t < M; ++t) gb[(F + lane) * GS + t] = 0.0;
-        }
-        wave_lds_sync();
-        // H = A_Fpi - G A_pipi / 2 on FP64 MFMA: tile I of free ranks, the
-        // m pivot columns (C = A_Fpi from hb, A = -G/2, B = A_pipi)
-        {
-            const int li = lane & 15, lg = lane >> 4;
-            constexpr int KH = (M + 3) / 4;
-#pragma unroll
-            for (int I = 0; I < T; ++I) {
-                ns_dbl4 c;
-#pragma unroll
-                for (int v = 0; v < 4; ++v)
-                    c[v] = li < M ? hb[(16 * I + lg + 4 * v) * GS + li] : 0.0;
-#pragma unroll
-                for (int k = 0; k < KH; ++k) {
-                    const int kk = 4 * k + lg;
-                    const double av = kk < M ? -0.5 * gb[(16 * I + li) * GS + kk] : 0.0;
-                    const double bv = (kk < M && li < M) ? app[kk * M + li] : 0.0;
-                    c = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, c, 0, 0, 0);
-                }
-                if (li < M) {
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) hb[(16 * I + lg + 4 * v) * GS + li] = c[v];
+            if (isfree) {
+                qf[rank] = qx;
+                qf[TP + rank] = qy;
+                if constexpr (DIM == 3) qf[2 * TP + rank] = qz;
+                fidx[rank] = lane;
+    #pragma unroll
+                for (int t = 0; t < M; ++t) {
+                    gb[rank * GS + t] = mono[t];
+                    hb[rank * GS + t] = akp[t];
                 }
             }
-        }
-        wave_lds_sync();
-        // t = b - A_:pi w_p at every node; the pivot nodes' values to LDS
-        double tk[R];
-#pragma unroll
-        for (int c = 0; c < R; ++c) {
-            double acc = bk[c];
-#pragma unroll
-            for (int u = 0; u < M; ++u) acc = fma(-akp[u], wpb[u * R + c], acc);
-            tk[c] = acc;
-        }
-        if (mypt < M) {
-#pragma unroll
-            for (int c = 0; c < R; ++c) tb[mypt * R + c] = tk[c];
-        }
-        wave_lds_sync();
-        if (isfree) {
-#pragma unroll
+
+            // ---- D: particular solution, t, y -----------------------------------
+            particular_and_h(lane, lb, rub, wpb, gb, hb, app);
+            // t = b - A_:pi w_p at every node; the pivot nodes' values to LDS
+            double tk[R];
+    #pragma unroll
             for (int c = 0; c < R; ++c) {
-                double acc = tk[c];
-#pragma unroll
-                for (int u = 0; u < M; ++u) acc = fma(-mono[u], tb[u * R + c], acc);
-                yb[rank * R + c] = acc;
+                double acc = bk[c];
+    #pragma unroll
+                for (int u = 0; u < M; ++u) acc = fma(-akp[u], wpb[u * R + c], acc);
+                tk[c] = acc;
+            }
+            if (mypt < M) {
+    #pragma unroll
+                for (int c = 0; c < R; ++c) tb[mypt * R + c] = tk[c];
+            }
+            wave_lds_sync();
+            if (isfree) {
+    #pragma unroll
+                for (int c = 0; c < R; ++c) {
+                    double acc = tk[c];
+    #pragma unroll
+                    for (int u = 0; u < M; ++u) acc = fma(-mono[u], tb[u * R + c], acc);
+                    yb[rank * R + c] = acc;
+                }
             }
         }
 
