@@ -276,11 +276,12 @@ struct NullSpaceMixed {
     // Phases A and D at two nodes per lane (node l and node 64 + l; N > 64):
     // gather, shift and scale; the polynomial block's LU with partial
     // pivoting over both node sets (the largest high word wins, the lowest
-    // node among equal ones); the kernel entries against the pivot nodes;
-    // G; the particular solution and H (rank-wise); t and the reduced
-    // right-hand sides y. Free nodes take ranks in node order. Leaves the
-    // LDS as the one-node-per-lane phases do and returns the scale and the
-    // rank-deficiency flag.
+    // node among equal ones); G; the kernel entries against the pivot
+    // nodes; the particular solution, t and H; the reduced right-hand sides
+    // y. Free nodes take ranks in node order. The phases run in the order
+    // that keeps the fewest values live (G goes to LDS before the kernel
+    // entries are formed, t is formed before the H product and y reads G
+    // back), and leave the LDS as the one-node-per-lane phases do.
     __device__ __forceinline__ static void node_phases_two(
         const int s, const int* __restrict__ ja_row, const double* __restrict__ x,
         const double* __restrict__ y, const double* __restrict__ z, const int lane, double* lds,
@@ -304,11 +305,8 @@ struct NullSpaceMixed {
         // gather, shift, scale; the radius from the largest squared distance
         // by its high word, as with one node per lane
         double qx[2], qy[2], qz[2], d2[2];
-        unsigned dkey[2];
         {
-            const int jn0 = ja_row[lane];
-            const int jn1 = has1 ? ja_row[64 + lane] : s;
-            const int jn[2] = {jn0, jn1};
+            const int jn[2] = {ja_row[lane], has1 ? ja_row[64 + lane] : s};
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
                 qx[k] = x[jn[k]] - x[s];
@@ -318,32 +316,23 @@ struct NullSpaceMixed {
                 if constexpr (DIM == 3) d2[k] = d2[k] + qz[k] * qz[k];
             }
         }
-        dkey[0] = static_cast<unsigned>(__double2hiint(d2[0]));
-        dkey[1] = has1 ? static_cast<unsigned>(__double2hiint(d2[1])) : 0u;
-        const unsigned dmax = wave_umax_bcast(dkey[0] > dkey[1] ? dkey[0] : dkey[1]);
-        const unsigned long long at0 = wave_ballot(dkey[0] == dmax);
+        const unsigned dkey0 = static_cast<unsigned>(__double2hiint(d2[0]));
+        const unsigned dkey1 = has1 ? static_cast<unsigned>(__double2hiint(d2[1])) : 0u;
+        const unsigned dmax = wave_umax_bcast(dkey0 > dkey1 ? dkey0 : dkey1);
+        const unsigned long long at0 = wave_ballot(dkey0 == dmax);
         double rho2 = at0 ? bcast(d2[0], __ffsll(at0) - 1)
-                          : bcast(d2[1], __ffsll(wave_ballot(dkey[1] == dmax)) - 1);
+                          : bcast(d2[1], __ffsll(wave_ballot(dkey1 == dmax)) - 1);
         if (!(rho2 > 0.0)) rho2 = 1.0;
         const double rho = sqrt(rho2);
         irho = wave_uniform(1.0 / rho);
         irho2 = irho * irho;
         double mono[2][M];
-        double bk[2][R];
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             qx[k] = qx[k] * irho;
             qy[k] = qy[k] * irho;
             qz[k] = qz[k] * irho;
             PolyBasis<P, DIM>{}(qx[k], qy[k], qz[k], mono[k], 1);
-            double r2 = qx[k] * qx[k] + qy[k] * qy[k];
-            if constexpr (DIM == 3) r2 = r2 + qz[k] * qz[k];
-            const double tq = rpow<Q - 2>(r2);
-            const double gq = static_cast<double>(Q) * tq;
-            bk[k][0] = static_cast<double>(Q * (Q + DIM - 2)) * tq;
-            bk[k][1] = -gq * qx[k];
-            bk[k][2] = -gq * qy[k];
-            if constexpr (DIM == 3) bk[k][3] = -gq * qz[k];
         }
 
         // A: LU with partial pivoting of P over both node sets
@@ -361,21 +350,30 @@ struct NullSpaceMixed {
             const unsigned long long c0 = act[0] & wave_ballot(key[0] == mk);
             const int ps = c0 ? 0 : 1;  // set of the pivot node
             const int p = __ffsll(c0 ? c0 : act[1] & wave_ballot(key[1] == mk)) - 1;
-            const double piv = bcast(ps ? mono[1][t] : mono[0][t], p);
+            // (selects of values: a select of the two elements' addresses
+            // would keep mono in scratch memory)
+            const double a0 = mono[0][t], a1 = mono[1][t];
+            const double piv = bcast(ps ? a1 : a0, p);
             if (!(piv != 0.0)) bad = true;
             double rcp = __builtin_amdgcn_rcp(piv);  // two Newton steps from the seed
             rcp = fma(rcp, fma(-piv, rcp, 1.0), rcp);
             rcp = fma(rcp, fma(-piv, rcp, 1.0), rcp);
+            // (inner loops over all M columns, guarded, so that they unroll
+            // fully and mono stays in registers)
             double pv[M];  // the pivot row right of column t
 #pragma unroll
-            for (int u = t + 1; u < M; ++u) pv[u] = bcast(ps ? mono[1][u] : mono[0][u], p);
+            for (int u = 0; u < M; ++u) {
+                const double u0 = mono[0][u], u1 = mono[1][u];
+                if (u > t) pv[u] = bcast(ps ? u1 : u0, p);
+            }
 #pragma unroll
             for (int k = 0; k < 2; ++k) {
                 const unsigned long long pm = ps == k ? 1ull << p : 0ull;
                 if (lane_in(act[k] & ~pm)) {
                     const double l = mono[k][t] * rcp;
 #pragma unroll
-                    for (int u = t + 1; u < M; ++u) mono[k][u] = fma(-l, pv[u], mono[k][u]);
+                    for (int u = 0; u < M; ++u)
+                        if (u > t) mono[k][u] = fma(-l, pv[u], mono[k][u]);
                     mono[k][t] = l;
                 }
                 mypt[k] = lane_in(pm) ? t : mypt[k];
@@ -384,14 +382,13 @@ struct NullSpaceMixed {
             if (lane == 0) rub[t] = rcp;
         }
         const bool isfree[2] = {lane_in(act[0]), lane_in(act[1])};
-        const int nfree0 = __popcll(act[0]);
-        const int rank[2] = {
-            static_cast<int>(__builtin_amdgcn_mbcnt_hi(
-                static_cast<unsigned>(act[0] >> 32),
-                __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(act[0]), 0u))),
-            nfree0 + static_cast<int>(__builtin_amdgcn_mbcnt_hi(
-                         static_cast<unsigned>(act[1] >> 32),
-                         __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(act[1]), 0u)))};
+        int rank[2];  // free nodes of the lanes below, then the whole first set
+#pragma unroll
+        for (int k = 0; k < 2; ++k)
+            rank[k] = static_cast<int>(__builtin_amdgcn_mbcnt_hi(
+                static_cast<unsigned>(act[k] >> 32),
+                __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(act[k]), 0u)));
+        rank[1] += static_cast<int>(__popcll(act[0]));
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             if (mypt[k] < M) {
@@ -404,7 +401,34 @@ struct NullSpaceMixed {
             }
         }
         wave_lds_sync();
-        // kernel entries against the pivot nodes
+        // G = L2 L1^-1 for both sets, to LDS by free rank
+#pragma unroll
+        for (int t = M - 2; t >= 0; --t) {
+            double acc[2] = {mono[0][t], mono[1][t]};
+#pragma unroll
+            for (int u = 0; u < M; ++u) {
+                if (u > t) {
+                    const double l1 = lb[u * M + t];
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) acc[k] = fma(-mono[k][u], l1, acc[k]);
+                }
+            }
+            mono[0][t] = acc[0];
+            mono[1][t] = acc[1];
+        }
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (isfree[k]) {
+                qf[rank[k]] = qx[k];
+                qf[TP + rank[k]] = qy[k];
+                if constexpr (DIM == 3) qf[2 * TP + rank[k]] = qz[k];
+                fidx[rank[k]] = lane + 64 * k;
+#pragma unroll
+                for (int t = 0; t < M; ++t) gb[rank[k] * GS + t] = mono[k][t];
+            }
+        }
+        // kernel entries against the pivot nodes: A_pipi, H's initial rows
+        // (A_Fpi by free rank)
         double akp[2][M];
 #pragma unroll
         for (int t = 0; t < M; ++t) {
@@ -418,6 +442,10 @@ struct NullSpaceMixed {
 #pragma unroll
                 for (int t = 0; t < M; ++t) app[mypt[k] * M + t] = akp[k][t];
             }
+            if (isfree[k]) {
+#pragma unroll
+                for (int t = 0; t < M; ++t) hb[rank[k] * GS + t] = akp[k][t];
+            }
         }
         if (lane < TP - F) {
 #pragma unroll
@@ -426,48 +454,30 @@ struct NullSpaceMixed {
             for (int t = 0; t < M; ++t) hb[(F + lane) * GS + t] = 0.0;
         }
         wave_lds_sync();
-        // G = L2 L1^-1, both sets
-#pragma unroll
-        for (int t = M - 2; t >= 0; --t) {
-            double acc[2] = {mono[0][t], mono[1][t]};
-#pragma unroll
-            for (int u = t + 1; u < M; ++u) {
-                const double l1 = lb[u * M + t];
-#pragma unroll
-                for (int k = 0; k < 2; ++k) acc[k] = fma(-mono[k][u], l1, acc[k]);
-            }
-            mono[0][t] = acc[0];
-            mono[1][t] = acc[1];
-        }
-#pragma unroll
-        for (int k = 0; k < 2; ++k) {
-            if (isfree[k]) {
-                qf[rank[k]] = qx[k];
-                qf[TP + rank[k]] = qy[k];
-                if constexpr (DIM == 3) qf[2 * TP + rank[k]] = qz[k];
-                fidx[rank[k]] = lane + 64 * k;
-#pragma unroll
-                for (int t = 0; t < M; ++t) {
-                    gb[rank[k] * GS + t] = mono[k][t];
-                    hb[rank[k] * GS + t] = akp[k][t];
-                }
-            }
-        }
 
-        // D: particular solution and H; t = b - A_:pi w_p at every node
-        particular_and_h(lane, lb, rub, wpb, gb, hb, app);
+        // D: the particular solution; t = b - A_:pi w_p at every node (the
+        // pivot nodes' values to LDS); H; y = t_F - G t_pi
+        particular_solution(lane, lb, rub, wpb);
         double tk[2][R];
 #pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            double r2 = qx[k] * qx[k] + qy[k] * qy[k];
+            if constexpr (DIM == 3) r2 = r2 + qz[k] * qz[k];
+            const double tq = rpow<Q - 2>(r2);
+            const double gq = static_cast<double>(Q) * tq;
+            tk[k][0] = static_cast<double>(Q * (Q + DIM - 2)) * tq;
+            tk[k][1] = -gq * qx[k];
+            tk[k][2] = -gq * qy[k];
+            if constexpr (DIM == 3) tk[k][3] = -gq * qz[k];
+        }
+#pragma unroll
         for (int c = 0; c < R; ++c) {
-            double acc[2] = {bk[0][c], bk[1][c]};
 #pragma unroll
             for (int u = 0; u < M; ++u) {
                 const double wp = wpb[u * R + c];
 #pragma unroll
-                for (int k = 0; k < 2; ++k) acc[k] = fma(-akp[k][u], wp, acc[k]);
+                for (int k = 0; k < 2; ++k) tk[k][c] = fma(-akp[k][u], wp, tk[k][c]);
             }
-            tk[0][c] = acc[0];
-            tk[1][c] = acc[1];
         }
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
@@ -476,7 +486,7 @@ struct NullSpaceMixed {
                 for (int c = 0; c < R; ++c) tb[mypt[k] * R + c] = tk[k][c];
             }
         }
-        wave_lds_sync();
+        h_product(lane, gb, hb, app);  // ends with a wave barrier: t_pi is in LDS
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             if (isfree[k]) {
@@ -484,22 +494,22 @@ struct NullSpaceMixed {
                 for (int c = 0; c < R; ++c) {
                     double acc = tk[k][c];
 #pragma unroll
-                    for (int u = 0; u < M; ++u) acc = fma(-mono[k][u], tb[u * R + c], acc);
+                    for (int u = 0; u < M; ++u)
+                        acc = fma(-gb[rank[k] * GS + u], tb[u * R + c], acc);
                     yb[rank[k] * R + c] = acc;
                 }
             }
         }
     }
 
-    // The rank-wise part of phase D: lane c < R solves P_pi^T w = g_c
-    // (U^T z = g_c forward, L1^T w = z back) into wpb; G's padding rows
-    // then take the place of P's pivot rows, and H = A_Fpi - G A_pipi / 2
-    // is formed on FP64 MFMA: tile I of free ranks, the m pivot columns
-    // (C = A_Fpi from hb, A = -G/2, B = A_pipi).
-    __device__ __forceinline__ static void particular_and_h(const int lane, const double* lb,
-                                                            const double* rub, double* wpb,
-                                                            double* gb, double* hb,
-                                                            const double* app) {
+    // The rank-wise parts of phase D. particular_solution: lane c < R
+    // solves P_pi^T w = g_c (U^T z = g_c forward, L1^T w = z back) into
+    // wpb. h_product: G's padding rows take the place of P's pivot rows,
+    // and H = A_Fpi - G A_pipi / 2 is formed on FP64 MFMA: tile I of free
+    // ranks, column tile J of the m pivot columns (C = A_Fpi from hb,
+    // A = -G/2, B = A_pipi).
+    __device__ __forceinline__ static void particular_solution(const int lane, const double* lb,
+                                                               const double* rub, double* wpb) {
 #pragma clang fp contract(off)
         if (lane < R) {
             using Mk = PolyDerivMasks<DIM, P>;
@@ -524,6 +534,9 @@ struct NullSpaceMixed {
             for (int t = 0; t < M; ++t) wpb[t * R + lane] = w[t];
         }
         wave_lds_sync();  // P's pivot rows read: G's padding rows take their place
+    }
+    __device__ __forceinline__ static void h_product(const int lane, double* gb, double* hb,
+                                                     const double* app) {
         if (lane < TP - F) {
 #pragma unroll
             for (int t = 0; t < M; ++t) gb[(F + lane) * GS + t] = 0.0;
@@ -631,7 +644,7 @@ struct NullSpaceMixed {
             constexpr unsigned long long kRows = N >= 64 ? ~0ull : ((1ull << N) - 1);
             unsigned long long act = kRows;
             int mypt = M;  // pivot step of this lane's node (M: a free node)
-    #pragma unroll
+#pragma unroll
             for (int t = 0; t < M; ++t) {
                 const double a = mono[t];
                 // pivot: the largest |a| among the rows still active, compared by
@@ -650,7 +663,7 @@ struct NullSpaceMixed {
                 const unsigned long long pm = 1ull << p;
                 if (lane_in(act & ~pm)) {
                     const double l = a * rcp;
-    #pragma unroll
+#pragma unroll
                     for (int u = t + 1; u < M; ++u) mono[u] = fma(-l, bcast(mono[u], p), mono[u]);
                     mono[t] = l;
                 }
@@ -666,7 +679,7 @@ struct NullSpaceMixed {
             // pivot rows: L1 below the diagonal, U on and above it; the pivot
             // nodes' points (in t_pi's place, which is written later)
             if (mypt < M) {
-    #pragma unroll
+#pragma unroll
                 for (int u = 0; u < M; ++u) lb[mypt * M + u] = mono[u];
                 pvn[mypt] = lane;
                 tb[mypt * R] = qx;
@@ -677,27 +690,27 @@ struct NullSpaceMixed {
             // kernel entries against the pivot nodes (the pivot lanes' values
             // form A_pipi)
             double akp[M];
-    #pragma unroll
+#pragma unroll
             for (int t = 0; t < M; ++t)
                 akp[t] = phi_pair(qx, qy, qz, tb[t * R], tb[t * R + 1], DIM == 3 ? tb[t * R + 2] : 0.0);
             if (mypt < M) {
-    #pragma unroll
+#pragma unroll
                 for (int t = 0; t < M; ++t) app[mypt * M + t] = akp[t];
             }
             // padding ranks F .. TP-1: zero points and H rows (G's below)
             if (lane < TP - F) {
-    #pragma unroll
+#pragma unroll
                 for (int a = 0; a < DIM; ++a) qf[a * TP + F + lane] = 0.0;
-    #pragma unroll
+#pragma unroll
                 for (int t = 0; t < M; ++t) hb[(F + lane) * GS + t] = 0.0;
             }
             wave_lds_sync();
             // G = L2 L1^-1: g L1 = l, from the last column (L1 read from LDS; the
             // pivot lanes compute values nobody reads)
-    #pragma unroll
+#pragma unroll
             for (int t = M - 2; t >= 0; --t) {
                 double acc = mono[t];
-    #pragma unroll
+#pragma unroll
                 for (int u = t + 1; u < M; ++u) acc = fma(-mono[u], lb[u * M + t], acc);
                 mono[t] = acc;
             }
@@ -706,7 +719,7 @@ struct NullSpaceMixed {
                 qf[TP + rank] = qy;
                 if constexpr (DIM == 3) qf[2 * TP + rank] = qz;
                 fidx[rank] = lane;
-    #pragma unroll
+#pragma unroll
                 for (int t = 0; t < M; ++t) {
                     gb[rank * GS + t] = mono[t];
                     hb[rank * GS + t] = akp[t];
@@ -714,26 +727,27 @@ struct NullSpaceMixed {
             }
 
             // ---- D: particular solution, t, y -----------------------------------
-            particular_and_h(lane, lb, rub, wpb, gb, hb, app);
+            particular_solution(lane, lb, rub, wpb);
+            h_product(lane, gb, hb, app);
             // t = b - A_:pi w_p at every node; the pivot nodes' values to LDS
             double tk[R];
-    #pragma unroll
+#pragma unroll
             for (int c = 0; c < R; ++c) {
                 double acc = bk[c];
-    #pragma unroll
+#pragma unroll
                 for (int u = 0; u < M; ++u) acc = fma(-akp[u], wpb[u * R + c], acc);
                 tk[c] = acc;
             }
             if (mypt < M) {
-    #pragma unroll
+#pragma unroll
                 for (int c = 0; c < R; ++c) tb[mypt * R + c] = tk[c];
             }
             wave_lds_sync();
             if (isfree) {
-    #pragma unroll
+#pragma unroll
                 for (int c = 0; c < R; ++c) {
                     double acc = tk[c];
-    #pragma unroll
+#pragma unroll
                     for (int u = 0; u < M; ++u) acc = fma(-mono[u], tb[u * R + c], acc);
                     yb[rank * R + c] = acc;
                 }
