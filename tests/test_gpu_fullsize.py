@@ -6,6 +6,10 @@ host kNN) against the CPU FP64 oracle run over all of them.
   1e-8 (max over stencils and operators of max_k |w - w_ref| / max_k |w_ref|).
 - n = 56, r^7 + P4 (71 x 71): mixed form within 1e-8, FP64 fallback rare.
 - D2Q9 interpolation (n = 9, r^3 + P2, nine centres): bit-identical.
+- 3-D n = 80, r^3 + P3 (100 x 100, 2,097,152 stencils of the 128^3 Morton
+  cube): the mixed form within 1e-8 and the FP64 form bit-identical, on a
+  131,072-stencil contiguous sample of the oracle (the whole cube takes the
+  oracle minutes).
 """
 import numpy as np
 import pytest
@@ -32,8 +36,9 @@ def _assemble(plan, W, centres=None):
     info = torch.full((nst,), -7, dtype=torch.int32, device="cuda:0")
     status = torch.full((nst,), 255, dtype=torch.uint8, device="cuda:0")
     cen = None if centres is None else tuple(_dev(c) for c in centres)
-    plan.assemble(nst, _dev(W["x"]), _dev(W["y"]), _dev(W["ja"]), w, centres=cen, info=info,
-                  status=status)
+    z = None if W.get("z") is None else _dev(W["z"])
+    plan.assemble(nst, _dev(W["x"]), _dev(W["y"]), _dev(W["ja"]), w, z=z, centres=cen,
+                  info=info, status=status)
     torch.cuda.synchronize()
     out = [t.cpu().numpy() for t in w]
     del w
@@ -108,3 +113,39 @@ def test_d2q9_interpolation_every_stencil_bit_identical():
     assert (iref == 0).all() and (info == 0).all() and (status == 0).all()
     for a, b in zip(w, wref):
         np.testing.assert_array_equal(a, b)
+
+
+@pytest.fixture(scope="module")
+def cube_n80():
+    from rbf_amd import workloads
+    W = workloads.build("cube_2m_n80")
+    assert W["nstencils"] == 1 << 21
+    s0, cnt = 1 << 20, 1 << 17  # a contiguous Morton block from the middle of the cube
+    wref, iref = O.assemble(3, 80, 3, 3, W["x"], W["y"], W["ja"], z=W["z"], s0=s0, count=cnt)
+    assert (iref == 0).all()
+    return W, wref, s0, cnt
+
+
+def test_cube_mixed_within_1e8(cube_n80):
+    W, wref, s0, cnt = cube_n80
+    plan = R.Plan(3, 80, 3, 3, "diff", precision="mixed", max_refine=3)
+    w, info, status = _assemble(plan, W)
+    assert (info == 0).all()
+    fb = (status & 4) != 0
+    sl = slice(s0 * 80, (s0 + cnt) * 80)
+    e = _relerr([a[sl] for a in w], wref, 80)
+    print(f"mixed 3-D n=80, 2M stencils, {plan.variant_name()}: max rel err {e.max():.3e} "
+          f"(sample of {cnt}), fallback {int(fb.sum())}, steps {np.bincount(status & 3, minlength=4)}")
+    assert e.max() <= MIXED_TOL
+    assert fb.mean() <= 1e-3
+
+
+def test_cube_fp64_bit_identical(cube_n80):
+    W, wref, s0, cnt = cube_n80
+    plan = R.Plan(3, 80, 3, 3, "diff")
+    w, info, status = _assemble(plan, W)
+    print(f"fp64 3-D n=80, 2M stencils, {plan.variant_name()}")
+    assert (info == 0).all() and (status == 0).all()
+    sl = slice(s0 * 80, (s0 + cnt) * 80)
+    for a, b in zip(w, wref):
+        np.testing.assert_array_equal(a[sl], b)
