@@ -213,11 +213,25 @@ struct NsListMixVariants {
          &launch_ns_list<DIM, N, P, Q, 64, 1>},
         {"workgroup nw=2 fp32 lu + fp64 refinement", &launch_group_mixed<DIM, N, P, Q>}};
 };
+// 3-D: the null-space forms only. The whole-system FP32 LU cannot tell an
+// exactly singular stencil (duplicated nodes) from the system's own small
+// pivots at FP32 resolution (kappa ~ 4e4), so a singular stencil could pass
+// as converged; the reduced system of the null-space form is definite and
+// well conditioned, and its pivot test flags it.
+template <int N, int P, int Q>
+struct NsListMixVariants<3, N, P, Q> {
+    static constexpr Variant list[2] = {
+        {"null space: fp64 poly lu + mfma schur, fp32 lu 16 symmetric steps, blocked-lu fallback",
+         &launch_ns_list<3, N, P, Q, 16, 1>},
+        {"null space: fp64 poly lu + mfma schur, fp32 symmetric lu, blocked-lu fallback",
+         &launch_ns_list<3, N, P, Q, 64, 1>}};
+};
 template <int DIM, int N, int P, int Q>
 constexpr Shape diffnsmix() {
     constexpr int M = npoly_dim(DIM, P);
-    return {DIM, N, P, Q, RBFA_OPS_DIFF, 0, RBFA_PREC_MIXED, N + M, DIM + 1,
-            NsListMixVariants<DIM, N, P, Q>::list, 3, true};
+    using L = NsListMixVariants<DIM, N, P, Q>;
+    return {DIM, N, P, Q, RBFA_OPS_DIFF, 0, RBFA_PREC_MIXED, N + M, DIM + 1, L::list,
+            static_cast<int>(sizeof(L::list) / sizeof(L::list[0])), true};
 }
 template <int DIM, int N, int P, int Q>
 constexpr Shape diffgroup() {

@@ -107,6 +107,5 @@ template hipError_t launch_ns_list<2, 56, 4, 7, 64, 1>(const LaunchArgs&);
 template hipError_t launch_ns_list<3, 80, 3, 3, 16, 1>(const LaunchArgs&);
 template hipError_t launch_ns_list<3, 80, 3, 3, 64, 1>(const LaunchArgs&);
 template hipError_t launch_group_mixed<2, 56, 4, 7>(const LaunchArgs&);
-template hipError_t launch_group_mixed<3, 80, 3, 3>(const LaunchArgs&);
 
 }  // namespace rbf_amd_internal

@@ -295,7 +295,8 @@ struct GroupMixed {
     static constexpr int off_xl = Base::lds_doubles;
     static constexpr int off_bc = off_xl + R * NT;
     static constexpr int off_red = off_bc + 2 * R;
-    static constexpr int lds_doubles = (off_red + 2 * NW * 2 * R + 1) / 2 * 2;
+    static constexpr int off_guard = off_red + 2 * NW * 2 * R;  // [2][NW]: max |entry|, tiny pivot
+    static constexpr int lds_doubles = (off_guard + 2 * NW + 1) / 2 * 2;
     static constexpr double kTol = 1e-10;
 
     // forward substitution with L (unit lower; rows pivotal after step K)
@@ -357,8 +358,33 @@ struct GroupMixed {
         const double rho = sqrt(rho2);
         float a[NC];
         Base::fill(a, g, xs, ys, zs, lds);
+        float rowmax = 0.f;
+        if (g < NT) {
+#pragma unroll
+            for (int j = 0; j < NT; ++j) rowmax = fmaxf(rowmax, fabsf(a[j]));
+        }
         ElimState<NT, float> st;
         Base::template elim_from<0, true>(a, st, lds, g, w);
+        // a pivot below 1e-6 of the largest matrix entry (duplicated nodes,
+        // nearly singular stencils): FP32 factors that small carry no
+        // information, and refinement can appear to converge on them (as
+        // in the one-wave whole-system form); the workgroup's verdict
+        // through LDS
+        double* guard = lds + off_guard;
+        bool tiny_pivot = false;
+        {
+            const unsigned wm = wave_umax(__float_as_uint(rowmax));
+            if (lane == 0) guard[w] = static_cast<double>(__uint_as_float(wm));
+            __syncthreads();
+            float smax = 0.f;
+#pragma unroll
+            for (int v = 0; v < NW; ++v) smax = fmaxf(smax, static_cast<float>(guard[v]));
+            const bool tiny = g < NT && !(fabsf(st.myrcp) * (1e-6f * smax) < 1.f);
+            if (lane == 0) guard[NW + w] = wave_ballot(tiny) ? 1.0 : 0.0;
+            __syncthreads();
+#pragma unroll
+            for (int v = 0; v < NW; ++v) tiny_pivot = tiny_pivot || guard[NW + v] != 0.0;
+        }
         double* bc = lds + off_bc;
         double* xl = lds + off_xl;
         float r32[R];
@@ -380,7 +406,7 @@ struct GroupMixed {
         float dprev[R];
 #pragma unroll
         for (int c = 0; c < R; ++c) dprev[c] = 0.f;
-        if (!st.info) {
+        if (!st.info && !tiny_pivot) {
             for (int it = 0; it < max_refine; ++it) {
                 if (st.mystep < NT) {
 #pragma unroll

@@ -122,6 +122,33 @@ def test_mixed_degenerate_stencils_fall_back(n, P, Q):
         assert e.max() <= MIXED_TOL
 
 
+@pytest.mark.parametrize("dim,m,n,P,Q", [(2, 24, 56, 4, 7), (3, 10, 80, 3, 3)])
+def test_large_mixed_degenerate_stencils_fall_back(dim, m, n, P, Q):
+    """The same for the systems above 64 unknowns (the 2-D n = 56 and the
+    3-D n = 80 null-space forms, two nodes per lane in 3-D): stencils made
+    singular by duplicated nodes go to the FP64 blocked-LU pass, whose info
+    is the oracle's; the rest stay within the mixed bar."""
+    pts = [np.array(a) for a in R.jittered_grid(dim, m, 0.25, seed=19)]
+    for k in (101, 402, 555):
+        for a in pts:
+            a[k + 1] = a[k]
+    ja = R.knn_stencils(n, *pts)
+    z = pts[2] if dim == 3 else None
+    wref, iref = O.assemble(dim, n, P, Q, pts[0], pts[1], ja, z=z)
+    bad = iref != 0
+    assert bad.any()
+    plan = R.Plan(dim, n, P, Q, "diff", precision="mixed", max_refine=3)
+    for v in range(plan.num_variants):
+        plan.variant = v
+        wm, im, st = _run(plan, pts[0], pts[1], ja, z=z)
+        e = _relerr(wm, wref, n)[~bad]
+        print(f"dim={dim} n={n} variant {v} ({plan.variant_name()}): {bad.sum()} singular "
+              f"stencils, fallback {((st & 4) != 0).sum()}, max rel err elsewhere {e.max():.3e}")
+        np.testing.assert_array_equal(im, iref)
+        assert ((st[bad] & 4) != 0).all(), plan.variant_name()
+        assert e.max() <= MIXED_TOL
+
+
 def test_mixed_null_space_converges_in_one_step():
     """The headline shape (n = 35, r^5 + P3) on the null-space variant: no
     stencil falls back and almost every stencil stops after one FP64
