@@ -220,9 +220,16 @@ struct NsListMixVariants {
 // well conditioned, and its pivot test flags it.
 template <int N, int P, int Q>
 struct NsListMixVariants<3, N, P, Q> {
-    static constexpr Variant list[2] = {
+    // the pivot row by v_readlane in every step of the 60 x 60 reduced LU
+    // first (2M stencils: 116.9 ms; 16 / 32 / all steps from the published
+    // column 119.5 / 136.7 / 140.7 ms)
+    static constexpr Variant list[4] = {
+        {"null space: fp64 poly lu + mfma schur, fp32 lu bc=readlane, blocked-lu fallback",
+         &launch_ns_list<3, N, P, Q, 0, 1>},
         {"null space: fp64 poly lu + mfma schur, fp32 lu 16 symmetric steps, blocked-lu fallback",
          &launch_ns_list<3, N, P, Q, 16, 1>},
+        {"null space: fp64 poly lu + mfma schur, fp32 lu 32 symmetric steps, blocked-lu fallback",
+         &launch_ns_list<3, N, P, Q, 32, 1>},
         {"null space: fp64 poly lu + mfma schur, fp32 symmetric lu, blocked-lu fallback",
          &launch_ns_list<3, N, P, Q, 64, 1>}};
 };

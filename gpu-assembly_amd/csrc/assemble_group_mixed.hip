@@ -106,6 +106,8 @@ template hipError_t launch_ns_list<2, 56, 4, 7, 16, 1>(const LaunchArgs&);
 template hipError_t launch_ns_list<2, 56, 4, 7, 64, 1>(const LaunchArgs&);
 template hipError_t launch_ns_list<3, 80, 3, 3, 16, 1>(const LaunchArgs&);
 template hipError_t launch_ns_list<3, 80, 3, 3, 64, 1>(const LaunchArgs&);
+template hipError_t launch_ns_list<3, 80, 3, 3, 32, 1>(const LaunchArgs&);
+template hipError_t launch_ns_list<3, 80, 3, 3, 0, 1>(const LaunchArgs&);
 template hipError_t launch_group_mixed<2, 56, 4, 7>(const LaunchArgs&);
 
 }  // namespace rbf_amd_internal
