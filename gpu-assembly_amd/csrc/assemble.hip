@@ -206,9 +206,16 @@ struct GroupVariants {
 };
 template <int DIM, int N, int P, int Q>
 struct NsListMixVariants {
-    static constexpr Variant list[3] = {
+    // the pivot row of the reduced LU by v_readlane in every step first
+    // (1M 2-D n = 56 stencils: 22.3 ms; 16 / 32 / all steps from the
+    // published column 22.9 / 25.0 / 23.0 ms)
+    static constexpr Variant list[5] = {
+        {"null space: fp64 poly lu + mfma schur, fp32 lu bc=readlane, blocked-lu fallback",
+         &launch_ns_list<DIM, N, P, Q, 0, 1>},
         {"null space: fp64 poly lu + mfma schur, fp32 lu 16 symmetric steps, blocked-lu fallback",
          &launch_ns_list<DIM, N, P, Q, 16, 1>},
+        {"null space: fp64 poly lu + mfma schur, fp32 lu 32 symmetric steps, blocked-lu fallback",
+         &launch_ns_list<DIM, N, P, Q, 32, 1>},
         {"null space: fp64 poly lu + mfma schur, fp32 symmetric lu, blocked-lu fallback",
          &launch_ns_list<DIM, N, P, Q, 64, 1>},
         {"workgroup nw=2 fp32 lu + fp64 refinement", &launch_group_mixed<DIM, N, P, Q>}};

@@ -104,6 +104,8 @@ hipError_t launch_ns_list(const LaunchArgs& a) {
 
 template hipError_t launch_ns_list<2, 56, 4, 7, 16, 1>(const LaunchArgs&);
 template hipError_t launch_ns_list<2, 56, 4, 7, 64, 1>(const LaunchArgs&);
+template hipError_t launch_ns_list<2, 56, 4, 7, 0, 1>(const LaunchArgs&);
+template hipError_t launch_ns_list<2, 56, 4, 7, 32, 1>(const LaunchArgs&);
 template hipError_t launch_ns_list<3, 80, 3, 3, 16, 1>(const LaunchArgs&);
 template hipError_t launch_ns_list<3, 80, 3, 3, 64, 1>(const LaunchArgs&);
 template hipError_t launch_ns_list<3, 80, 3, 3, 32, 1>(const LaunchArgs&);
