@@ -477,7 +477,12 @@ RBFA_API int32_t rbfa_batch_class_count(const rbfa_batch* b, int32_t cls) {
 
 RBFA_API int32_t rbfa_batch_set_mode(rbfa_batch* b, int32_t mode) {
     if (!b || (mode != 0 && mode != 1)) return RBFA_ERR_ARG;
-    if (mode == 1 && !kPersistentEnabled) return RBFA_ERR_UNSUPPORTED;
+    if (mode == 1 && !kPersistentEnabled) {
+        rbf_amd_internal::set_last_error(
+            "rbfa_batch_set_mode: the persistent launch (mode 1) is switched off in this build; "
+            "mode 0 runs one launch per size class in the captured graph");
+        return RBFA_ERR_UNSUPPORTED;
+    }
     b->mode = mode;
     return RBFA_OK;
 }
@@ -574,7 +579,11 @@ RBFA_API const char* rbfa_batch_class_variant_name(const rbfa_batch* b, int32_t 
 
 RBFA_API int32_t rbfa_batch_set_class_variant(rbfa_batch* b, int32_t cls, int32_t v) {
     const int u = batch_slot(b, cls);
-    if (u < 0 || v < 0 || v >= kClasses[b->cls[u]].nvariants) return RBFA_ERR_ARG;
+    if (u < 0 || v < 0 || v >= kClasses[b->cls[u]].nvariants) {
+        rbf_amd_internal::set_last_error("rbfa_batch_set_class_variant: no such class or variant "
+                       "(rbfa_batch_class_num_variants lists them)");
+        return RBFA_ERR_ARG;
+    }
     if (b->variant[u] != v) {
         b->variant[u] = v;
         drop_graphs(b);  // the next launch captures the pipeline again

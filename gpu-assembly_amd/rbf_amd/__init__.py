@@ -420,8 +420,8 @@ class Batch:
         return None if r is None else r.decode()
 
     def set_class_variant(self, c, v):
-        _check(lib().rbfa_batch_set_class_variant(self._h, int(c), int(v)),
-               "rbfa_batch_set_class_variant")
+        _io_check(lib().rbfa_batch_set_class_variant(self._h, int(c), int(v)),
+                  "rbfa_batch_set_class_variant")
 
     @property
     def mode(self):
@@ -429,7 +429,7 @@ class Batch:
 
     @mode.setter
     def mode(self, m):
-        _check(lib().rbfa_batch_set_mode(self._h, int(m)), "rbfa_batch_set_mode")
+        _io_check(lib().rbfa_batch_set_mode(self._h, int(m)), "rbfa_batch_set_mode")
 
     def launch(self, stream=None):
         import torch

@@ -310,8 +310,9 @@ def test_uneven_batch_autotune():
     ms = b.autotune(reps=2)
     assert len(ms) == 2 and ms[0] > 0
     assert ms[1] == -1.0 and b.mode == 0  # the persistent mode is switched off
-    with pytest.raises(R.RbfError):
+    with pytest.raises(R.RbfError) as e:
         b.mode = 1
+    assert e.value.status == R.ERR_UNSUPPORTED and "switched off" in str(e.value)
     assert all(b.class_variant(c) for c in range(4))
 
 
