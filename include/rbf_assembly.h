@@ -298,7 +298,9 @@ RBFA_API int32_t rbfa_plan_autotune(rbfa_plan* plan, int32_t nstencils, const do
  * (w: host array of 3 device pointers laid out like ja); launch replays
  * it. Mode 0: one launch per class; mode 1: one persistent launch over
  * size-bucketed work queues, currently switched off (set_mode(1) returns
- * RBFA_ERR_UNSUPPORTED). autotune picks each class's kernel variant and
+ * RBFA_ERR_UNSUPPORTED and leaves the reason in rbfa_last_error; so does
+ * set_class_variant, with RBFA_ERR_ARG, for a class or variant past the
+ * lists below). autotune picks each class's kernel variant and
  * the faster enabled mode (ms_out[2] = mode 0, mode 1 milliseconds; -1 for
  * a mode that is switched off). Calls that overlap on one batch race on
  * its work-queue counters.
