@@ -10,6 +10,8 @@ host kNN) against the CPU FP64 oracle run over all of them.
   cube): the mixed form within 1e-8 and the FP64 form bit-identical, on a
   131,072-stencil contiguous sample of the oracle (the whole cube takes the
   oracle minutes).
+- The uneven batch (graded disk, 2,008,000 stencils in four size classes,
+  ragged CSR, captured hipGraph): bit-identical over every stencil.
 """
 import numpy as np
 import pytest
@@ -149,3 +151,37 @@ def test_cube_fp64_bit_identical(cube_n80):
     sl = slice(s0 * 80, (s0 + cnt) * 80)
     for a, b in zip(w, wref):
         np.testing.assert_array_equal(a[sl], b)
+
+
+def test_disk_uneven_batch_every_stencil_bit_identical():
+    """The uneven batch at its benchmarked size: the graded unit disk of
+    2,000,000 rejection-sampled interior nodes (spacing 1:4 from the centre
+    to the rim) and 8,000 rim nodes, four stencil-size classes in one ragged
+    CSR graph, run as the captured hipGraph of per-class launches (each
+    class's default shape), against the oracle over every stencil."""
+    import torch
+    from rbf_amd import workloads
+    W = workloads.build_disk()
+    ia, ja, x, y = W["ia"], W["ja"], W["x"], W["y"]
+    assert x.size == 2_008_000
+    nnz = int(ia[-1])
+    wref = [np.zeros(nnz) for _ in range(3)]
+    iref = np.zeros(x.size, np.int32)
+    for (n, P, Q) in workloads.DISK_CLASSES:
+        rows = np.nonzero(W["sizes"] == n)[0].astype(np.int32)
+        assert rows.size > 0
+        O.assemble_rows(2, n, P, Q, rows, ia, x, y, ja, wref, iref)
+    assert (iref == 0).all()
+    b = R.Batch(workloads.DISK_CLASSES)
+    w = [torch.full((nnz,), np.nan, dtype=torch.float64, device="cuda:0") for _ in range(3)]
+    info = torch.full((x.size,), -1, dtype=torch.int32, device="cuda:0")
+    b.bind(ia, _dev(x), _dev(y), _dev(ia), _dev(ja), w, info=info)
+    counts = [b.class_count(c) for c in range(len(workloads.DISK_CLASSES))]
+    assert sum(counts) == x.size
+    b.launch()
+    torch.cuda.synchronize()
+    print(f"disk, {x.size} stencils, class counts {counts}, "
+          f"{[b.class_variant(c) for c in range(len(counts))]}")
+    assert (info.cpu().numpy() == 0).all()
+    for a, r in zip(w, wref):
+        np.testing.assert_array_equal(a.cpu().numpy(), r)
