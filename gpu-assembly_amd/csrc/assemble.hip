@@ -299,7 +299,9 @@ const Shape kShapes[] = {
     interp64<9, 2, 3, 9>(), interp64<21, 2, 3, 9>(),
     diffmix<2, 21, 2, 3>(), diffmix<2, 35, 3, 5>(), diffmix<2, 13, 2, 3>(),
     diffmix<2, 35, 3, 3>(), diffmix<2, 21, 2, 5>(), diffmix<2, 13, 1, 3>(),
-    // systems larger than a wavefront: one two-wave workgroup per stencil
+    // systems larger than a wavefront: FP64 as the blocked LU (a four-wave
+    // workgroup per stencil, the two-wave row-per-lane engine as its second
+    // variant); mixed as the null-space form, one wave per stencil
     diffgroup<2, 56, 4, 7>(), diffgroup<3, 80, 3, 3>(),
     diffnsmix<2, 56, 4, 7>(), diffnsmix<3, 80, 3, 3>(),
 };
